@@ -1,0 +1,245 @@
+"""Benchmark: 800x800 fisheye depth maps per second on one MI355X.
+
+Workload (default): one seeded synthetic pair at 800x800 (f = 800/pi, 180 deg FOV,
+baseline 0.3 m along x, rotation <= 2 deg), 5 pyramid levels (ratio 2) x 20 warps x
+50 primal-dual iterations, lambda = 50, alpha0 = 5, alpha1 = 1, beta = 10,
+gamma = 0.5, fp32.  A step is one VFS.solve_stereo (uint8 images already on the
+GPU -> flow) followed by triangulate_flow (-> depth); the timed region covers K
+steps between device synchronisations.
+
+Prints one JSON line with the throughput, the roofline of the dominant kernel
+(measured with HIP events on the launch stream), and the float64 CPU oracle
+timed on the host beside it.
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "vfs-python_amd")
+for _p in (PKG, ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "800² depth maps/s & solver HBM GB/s; batched/adjoint vs plain at same error"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per pixel of a level in the per-kernel formulation.
+B_DUAL, B_L1, B_PRIMAL = 81, 41, 77
+B_ITER = B_DUAL + B_L1 + B_PRIMAL
+B_WARP = 150
+B_LEVEL = 38
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup_solver(name, seed, solver):
+    import torch
+    import vfs as V
+    import vfs_params as P
+    import vector_field as VF
+    import equidistant_cam as E
+    import synthetic as S
+
+    pair = S.workload_pair(name, seed=seed)
+    cam = E.EquidistantCam(pair["cam"])
+    cam.setup_module()
+    Hh, Ww = pair["img_ref"].shape
+    tvf, cvf = VF.get_vector_fields(VF.get_grid_zero((Hh, Ww)), cam, cam, pair["R"], pair["t"], 1,
+                                    VF.unit_normalizer)
+    tvf = S.sanitize_field(tvf)
+    prm = P.vfs_params(pair["img_ref"])
+    S.apply_workload_params(prm, name)
+    v = V.VFS(prm)
+    v.solver = solver
+    v.setup_module()
+    v.compute_pyramid_sizes()
+    v.create_mask_pyramid(pair["mask"])
+    v.create_vector_field_pyramid(tvf, cvf)
+    img_ref = torch.as_tensor(pair["img_ref"], device="cuda")
+    img_other = torch.as_tensor(pair["img_other"], device="cuda")
+    return dict(pair=pair, cam=cam, vfs=v, prm=prm, tvf=tvf, cvf=cvf, img_ref=img_ref, img_other=img_other)
+
+
+def dual_kernel_roofline(sizes, n_warp, n_iter, prm):
+    """Average duration and algorithmic bandwidth of vfs_update_dual over the launch mix of one solve.
+
+    Every level's n_warp * n_iter launches are replayed at that level's size on the
+    current stream, bracketed by HIP events (torch.cuda.Event on the same stream).
+    """
+    import ctypes
+    import torch
+    import vfs_hip as H
+    k = H.make_params(prm)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    total_ms, total_launches, total_bytes = 0.0, 0, 0
+    per_level = []
+    s = H.stream()
+    for (hh, ww) in sizes:
+        n = n_warp * n_iter
+        p = torch.zeros((hh, ww, 2), device="cuda")
+        q = torch.zeros((hh, ww, 4), device="cuda")
+        tp = torch.empty((hh, ww, 2), device="cuda")
+        u = torch.rand((hh, ww), device="cuda", generator=g)
+        vb = torch.rand((hh, ww, 2), device="cuda", generator=g) * 0.01
+        abc = torch.rand((hh, ww, 3), device="cuda", generator=g)
+        mask = (torch.rand((hh, ww), device="cuda", generator=g) > 0.2).to(torch.uint8)
+        args = (H.ptr(p), H.ptr(tp), H.ptr(q), 1.0, H.ptr(u), H.ptr(vb), H.ptr(abc), H.ptr(mask), hh, ww,
+                ctypes.byref(k))
+        fn = H.lib().vfs_update_dual
+        for _ in range(5):
+            fn(s, *args)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            rc = fn(s, *args)
+        e1.record()
+        torch.cuda.synchronize()
+        H.check(rc, "vfs_update_dual")
+        ms = e0.elapsed_time(e1)
+        per_level.append(dict(size=[hh, ww], launches=n, avg_us=1e3 * ms / n,
+                              gbs=B_DUAL * hh * ww * n / (ms * 1e-3) / 1e9))
+        total_ms += ms
+        total_launches += n
+        total_bytes += B_DUAL * hh * ww * n
+    avg_s = total_ms * 1e-3 / total_launches
+    bytes_per_launch = total_bytes / total_launches
+    return dict(avg_us=avg_s * 1e6, bytes_per_launch=bytes_per_launch,
+                achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
+
+
+def cpu_oracle_baseline(ctx, name):
+    """float64 NumPy oracle (1 thread) on the full pyramid with 1 of nWarpIters warps per level."""
+    from oracle import vfs_oracle as O
+    import synthetic as S
+    w = S.WORKLOADS[name]
+    pair = ctx["pair"]
+    Hh, Ww = pair["img_ref"].shape
+    oprm = O.OracleParams(Hh, Ww, nLevel=w["nLevel"], fScale=2.0, nWarpIters=1, nSolverIters=w["nSolverIters"],
+                          Lambda=w["Lambda"], alpha0=w["alpha0"], alpha1=w["alpha1"], beta=w["beta"],
+                          gamma=w["gamma"])
+    t0 = time.perf_counter()
+    O.solve_stereo(pair["img_ref"], pair["img_other"], pair["mask"], ctx["tvf"], ctx["cvf"], oprm)
+    dt = time.perf_counter() - t0
+    full = dt * w["nWarpIters"]
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = os.cpu_count()
+    cpu = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=1.0 / full, unit="depth maps/s", cores=1, kind="port",
+                sample=(f"float64 NumPy oracle, 1 thread, {Hh}x{Ww} {w['nLevel']}-level pyramid with 1 of "
+                        f"{w['nWarpIters']} warps per level ({w['nSolverIters']} PD iterations): {dt:.1f} s, "
+                        f"extrapolated x{w['nWarpIters']} to one map"),
+                host=dict(cpu_count=os.cpu_count(), affinity=aff, cpu_model=cpu))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="plain_full_800")
+    ap.add_argument("--solver", default="graph", choices=["graph", "eager"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every pair is independent and this build runs on a single MI355X; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import synthetic as S
+    torch.cuda.set_device(0)
+    w = S.WORKLOADS[a.workload]
+    log(f"[bench] workload {a.workload}: generating the pair and vector fields")
+    ctx = setup_solver(a.workload, a.seed, a.solver)
+    v = ctx["vfs"]
+    sizes = v.sizes
+    px = sum(h * ww for h, ww in sizes)
+    px_iter = px * w["nWarpIters"] * w["nSolverIters"]
+    bytes_map = sum(h * ww * (w["nWarpIters"] * (w["nSolverIters"] * B_ITER + B_WARP) + B_LEVEL) for h, ww in sizes)
+
+    def step():
+        v.solve_stereo(ctx["img_ref"], ctx["img_other"])
+        return v.triangulate_flow(ctx["cam"], ctx["pair"]["t"])
+
+    log(f"[bench] warmup {a.warmup}")
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] timing {a.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        xyz = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    maps_s = a.steps / dt
+
+    # accuracy of the timed result against the synthetic ground truth (not a parity check)
+    pair = ctx["pair"]
+    m = pair["mask"] > 0
+    z = xyz[..., 2].cpu().numpy()[m]
+    zg = pair["z_gt"][m]
+    ok = np.isfinite(z) & (zg > 1e-3)
+    depth_err = float(np.median(np.abs(z[ok] - zg[ok]) / zg[ok]))
+
+    roof = None
+    if not a.no_roofline:
+        log("[bench] dual-kernel roofline pass (HIP events)")
+        r = dual_kernel_roofline(sizes, w["nWarpIters"], w["nSolverIters"], v.param)
+        roof = dict(bound="hbm", kernel="k_update_dual", achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
+                    per_level=r["per_level"])
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("[bench] CPU oracle baseline (bounded sample)")
+        cpu = cpu_oracle_baseline(ctx, a.workload)
+
+    out = {
+        "metric": METRIC,
+        "value": round(maps_s, 3),
+        "unit": "depth maps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * dt / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "size": [w["H"], w["W"]], "levels": w["nLevel"],
+                   "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
+                   "solver": f"plain per-kernel launches ({a.solver})", "pairs": 1},
+        "mpx_iter_per_s": round(px_iter * maps_s / 1e6, 1),
+        "solver_effective_gbs": round(bytes_map * maps_s / 1e9, 1),
+        "median_rel_depth_err_vs_gt": round(depth_err, 4),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+/*
+ * vfs_hip.h -- C ABI of libvfs_hip.so, the gfx950 implementation of the VFS
+ * coarse-to-fine variational fisheye stereo solver.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, VFS_EINVAL (-22) for bad sizes or
+ *     null pointers, or a positive hipError_t from the launch.
+ *   - `stream` is a hipStream_t passed as void* (0 = the null stream).  Work is
+ *     enqueued and the function returns without synchronising.
+ *   - All arrays are device pointers to C-contiguous float32 (H, W[, C]) or
+ *     uint8 (H, W) buffers owned by the caller.  The library never allocates
+ *     device memory for the caller's data and keeps no pointer after returning
+ *     (the plan objects below keep the pointers they were built with).
+ *   - "Sampled" inputs, which the reference passes as CUDA texture objects
+ *     (Mirror addressing, normalised coordinates), are passed as a raw array
+ *     plus its own size (Hs, Ws); the sampling semantics are applied in the
+ *     kernel: bilinear with exact weights, half-sample symmetric mirror edges.
+ *   - Mask values are uint8; 0 = outside.  Stencil neighbours outside the image
+ *     are read through the mirror (index -1 -> 0, N -> N-1).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * eshibusawa/VFS-Python).
+ */
+#ifndef VFS_HIP_H_
+#define VFS_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFS_EINVAL (-22)
+
+/* Solver parameters: runtime values of the constants that Python/vfs.py:52-68
+ * substitutes into the CUDA source before NVRTC compilation. */
+typedef struct vfs_params {
+    float beta;          /* VFS_BETA          diffusion weight exp(-beta |g|^gamma) */
+    float gamma;         /* VFS_GAMMA */
+    float alpha0;        /* VFS_ALPHA0        TGV second-order weight */
+    float alpha1;        /* VFS_ALPHA1        TGV first-order weight */
+    float eta_p;         /* VFS_ETA_P = 3 */
+    float eta_q;         /* VFS_ETA_Q = 2 */
+    float lambda;        /* VFS_LAMBDA        data term weight */
+    float limit_range;   /* VFS_UPPER_LIMIT   upper clamp of the per-warp increment */
+    float f_scale;       /* VFS_FSCALE        pyramid ratio, flow scale on upsampling */
+    float min_grad;      /* VFS_MIN_NORMALIZED_GRAD = 1e-8 */
+    float min_tensor;    /* VFS_MIN_TESOR_VALUE = 1e-8 */
+    float a11;           /* float(alpha1 * alpha1) evaluated in double (Python/vfs.cu:231) */
+    float a004;          /* float(4 * alpha0 * alpha0) evaluated in double (Python/vfs.cu:230) */
+    int32_t median_k;    /* VFS_MEDIAN_FILTER_KERNEL_SIZE, 3 or 5 */
+} vfs_params;
+
+/* ---- pyramids (Python/vfs.py:93-218) ------------------------------------------ */
+
+/* castAndScale, Python/vfs.cu:145-156: out = float(in) / 256. */
+int vfs_cast_and_scale(void* stream, float* out, const uint8_t* in, int H, int W);
+
+/* downSamplingImage, Python/vfs.cu:171-174 (Bilinear sampler :35-49). */
+int vfs_downsample_image(void* stream, float* out, int Hd, int Wd,
+                         const float* src, int Hs, int Ws);
+
+/* downSamplingVectorField, Python/vfs.cu:181-184; float2 elements. */
+int vfs_downsample_vector_field(void* stream, float* out, int Hd, int Wd,
+                                const float* src, int Hs, int Ws);
+
+/* downSamplingMask, Python/vfs.cu:176-179 (NearestNeighbor :54-59):
+ * out[Y,X] = src[((2Y+1)Hs) / (2Hd), ((2X+1)Ws) / (2Wd)] (integer division). */
+int vfs_downsample_mask(void* stream, uint8_t* out, int Hd, int Wd,
+                        const uint8_t* src, int Hs, int Ws);
+
+/* ---- per-level and per-warp kernels ------------------------------------------ */
+
+/* warpingImage, Python/vfs.cu:186-202: out = S_lin(src, X + warp.x, Y + warp.y).
+ * warp is (H, W, 2); src is (Hs, Ws). */
+int vfs_warp_image(void* stream, float* out, const float* warp, int H, int W,
+                   const float* src, int Hs, int Ws);
+
+/* gaussianMasked, Python/vfs.cu:204-226: binomial 3x3 / 16, 0 where mask == 0. */
+int vfs_gaussian_masked(void* stream, float* out, const float* src, const uint8_t* mask,
+                        int H, int W);
+
+/* computeTensorEtaMasked, Python/vfs.cu:239-281: abc, eta are (H, W, 3). */
+int vfs_compute_tensor_eta(void* stream, float* abc, float* eta, const float* img,
+                           const uint8_t* mask, int H, int W, const vfs_params* prm);
+
+/* findWarpingVector, Python/vfs.cu:283-302: out = S_lin(tvf, X + warp.x, Y + warp.y);
+ * out, warp (H, W, 2); tvf (Hs, Ws, 2). */
+int vfs_find_warping_vector(void* stream, float* out, const float* warp, int H, int W,
+                            const float* tvf, int Hs, int Ws);
+
+/* computeDerivatives, Python/vfs.cu:304-324: out (H, W, 2) = (Iu, Iz); ref, other
+ * (H, W); tvf (H, W, 2). */
+int vfs_compute_derivatives(void* stream, float* out, const float* ref, const float* other,
+                            const float* tvf, int H, int W);
+
+/* updateDualVariables, Python/vfs.cu:326-419.  p (H,W,2) and q (H,W,4) are
+ * updated in place; tp (H,W,2) is written; u_bar (H,W), v_bar (H,W,2),
+ * abc (H,W,3) are read. */
+int vfs_update_dual(void* stream, float* p, float* tp, float* q, float sigma,
+                    const float* u_bar, const float* v_bar, const float* abc,
+                    const uint8_t* mask, int H, int W, const vfs_params* prm);
+
+/* l1Thresholding, Python/vfs.cu:421-505: out (H, W) = thresholded u.  out may
+ * alias u (each pixel reads only its own u). */
+int vfs_l1_thresholding(void* stream, float* out, float tau, const float* tp,
+                        const float* u, const float* u_bar, const float* deriv,
+                        const float* eta, const uint8_t* mask, int H, int W,
+                        const vfs_params* prm);
+
+/* updatePrimalVariables, Python/vfs.cu:507-581: writes u_bar_out (H, W) and
+ * v_bar_out (H, W, 2); they may alias u_bar / v_bar. */
+int vfs_update_primal(void* stream, float* u_bar_out, float* v_bar_out, float tau, float mu,
+                      const float* p, const float* q, const float* u, const float* u_bar,
+                      const float* v_bar, const float* abc, const float* eta,
+                      const uint8_t* mask, int H, int W, const vfs_params* prm);
+
+/* medianFilter, Python/vfs.cu:583-639: k x k median, clamp-to-edge, masked
+ * pixels copy their input.  ksize is 3 or 5; out must not alias in. */
+int vfs_median_filter(void* stream, float* out, const float* in, const uint8_t* mask,
+                      int H, int W, int ksize);
+
+/* computeOpticalFlowWithClamp, Python/vfs.cu:641-676: in place on flow (H,W,2)
+ * and u0 (H,W): d = min(uf - u0, limit); u0 += d; flow += d * tv. */
+int vfs_optical_flow_with_clamp(void* stream, float* flow, float* u0, const float* uf,
+                                const float* tv, const uint8_t* mask, int H, int W,
+                                float limit_range);
+
+/* upSamplingVectorField1 / upSamplingVectorField, Python/vfs.cu:678-705:
+ * out = f_scale * S_lin(src, (X + 0.5) Ws / Wd - 0.5, ...); channels is 1 or 2. */
+int vfs_upsample(void* stream, float* out, int Hd, int Wd, const float* src, int Hs, int Ws,
+                 int channels, float f_scale);
+
+/* triangulateRays, Python/vfs.cu:707-745: out (H,W,3); rays (H,W,3); t (3) is a
+ * device pointer. */
+int vfs_triangulate_rays(void* stream, float* out, const float* rays_ref,
+                         const float* rays_other, const float* t, const uint8_t* mask,
+                         int H, int W);
+
+/* Fill helpers used by the solver loop (the reference uses cp.zeros / cp.copy /
+ * flow[mask == 0] = 0 at Python/vfs.py:489-497,531). */
+int vfs_fill(void* stream, float* out, float value, int64_t n);
+int vfs_copy(void* stream, float* out, const float* in, int64_t n);
+int vfs_zero_masked(void* stream, float* flow, const uint8_t* mask, int H, int W, int channels);
+
+/* ---- equidistant camera (Python/equidistant_cam.cuh, Python/template_cam.cu) --- */
+
+/* getRays, Python/template_cam.cu:25-37: out (H,W,3) = unproject((X, Y)). */
+int vfs_cam_get_rays(void* stream, float* out, int H, int W, float f, float u0, float v0);
+/* getUVFromXYZ, Python/template_cam.cu:56-70: uv (H,W,2) = project(xyz (H,W,3)). */
+int vfs_cam_get_uv_from_xyz(void* stream, float* uv, const float* xyz, int H, int W,
+                            float f, float u0, float v0);
+/* getXYZFromUV, Python/template_cam.cu:72-86: xyz (H,W,3) = unproject(uv (H,W,2)). */
+int vfs_cam_get_xyz_from_uv(void* stream, float* xyz, const float* uv, int H, int W,
+                            float f, float u0, float v0);
+
+/* getThetas, Python/template_cam.cu:39-54: out (H,W,3) = (atan2(x,z), atan2(y,z),
+ * atan2(|xy|,z)) of rays (H,W,3). */
+int vfs_cam_get_thetas(void* stream, float* out, const float* rays, int H, int W);
+/* getXYZFromDistance, Python/template_cam.cu:88-112: out (H,W,3) = dist * ray, or
+ * (0,0,-1) where mask == 0. */
+int vfs_cam_xyz_from_distance(void* stream, float* out, const float* rays, const float* dist,
+                              const uint8_t* mask, int H, int W);
+
+/* get_vector_fields after the normaliser, Python/vector_field.py:34-54, fused per
+ * pixel: tvf = unit((project_ref(xyz + st0) - project_ref(xyz - st1)) / 2) and
+ * cvf = project_other(R xyz) - uv.  tvf, cvf, uv are (H,W,2); xyz is (H,W,3).
+ * cam_ref / cam_other = (f, u0, v0), R row-major 3x3, st0 / st1 the scaled
+ * translations (3): host pointers. */
+int vfs_vector_fields(void* stream, float* tvf, float* cvf, const float* xyz, const float* uv,
+                      int H, int W, const float* cam_ref, const float* cam_other,
+                      const float* R, const float* st0, const float* st1);
+
+/* ---- one pyramid level of VFS.solve_stereo (Python/vfs.py:473-521) ------------- */
+
+/* Buffers of one pyramid level.  Inputs: ref, other (H,W), tvf (H,W,2), mask.
+ * State (in/out): flow (H,W,2), u (H,W), u_bar (H,W).  The remaining fields are
+ * caller-provided workspace of the stated shapes. */
+typedef struct vfs_level {
+    int32_t H, W;
+    const float* ref;
+    const float* other;
+    const float* tvf;
+    const uint8_t* mask;
+    float* flow;
+    float* u;
+    float* u_bar;
+    float* ref_smoothed;  /* (H,W)   */
+    float* abc;           /* (H,W,3) */
+    float* eta;           /* (H,W,3) */
+    float* tv;            /* (H,W,2) */
+    float* warped;        /* (H,W)   */
+    float* deriv;         /* (H,W,2) */
+    float* p;             /* (H,W,2) */
+    float* q;             /* (H,W,4) */
+    float* tp;            /* (H,W,2) */
+    float* v_bar;         /* (H,W,2) */
+    float* u0;            /* (H,W)   */
+    float* uf;            /* (H,W)   */
+} vfs_level;
+
+/* Runs the warp loop of one level with the reference's kernel decomposition
+ * (Gaussian + tensor once, then per warp: warping vector, warp, derivatives,
+ * reset, n_iter x (dual, threshold, primal), median, clamp), enqueued kernel by
+ * kernel on `stream`.  taus/sigmas/mus are host arrays of n_iter values. */
+int vfs_solve_level_plain(void* stream, const vfs_level* lv, const vfs_params* prm,
+                          const float* taus, const float* sigmas, const float* mus,
+                          int n_iter, int n_warp);
+
+/* The same level loop recorded once into a HIP graph and replayed.  A plan
+ * keeps the pointers of `lv` and the schedule; launching it re-runs the level
+ * on `stream`.  Returns an opaque handle in *plan. */
+int vfs_plan_create_level_plain(void** plan, const vfs_level* lv, const vfs_params* prm,
+                                const float* taus, const float* sigmas, const float* mus,
+                                int n_iter, int n_warp);
+int vfs_plan_launch(void* plan, void* stream);
+
+/* Generic recording: vfs_capture_begin creates a stream in capture mode
+ * (thread-local) and returns it; every vfs_* launch enqueued on it by this
+ * thread is recorded.  vfs_capture_end stops the capture, destroys the stream
+ * and instantiates the recording as a plan for vfs_plan_launch.  Used to
+ * record a whole solve_stereo (pyramids, all levels, upsampling) as one graph.
+ * No reference counterpart: the reference synchronises after every launch. */
+int vfs_capture_begin(void** capture_stream);
+int vfs_capture_end(void* capture_stream, void** plan);
+int vfs_plan_destroy(void* plan);
+
+/* Library version string (static storage). */
+const char* vfs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFS_HIP_H_ */

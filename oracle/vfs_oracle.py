@@ -1,0 +1,600 @@
+"""Float64 NumPy restatement of the VFS coarse-to-fine fisheye stereo solver.
+
+TEST INFRASTRUCTURE ONLY.  This module is the correctness oracle and the CPU
+baseline: only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` may import it, and only as the checker.  The product path
+(``vfs-python_amd``) never imports it.
+
+Every function restates one kernel (or host routine) of the reference in
+float64, vectorised over pixels, and cites the file:line it follows
+(``Python/vfs.cu``, ``Python/vfs.py``, ``Python/equidistant_cam.cuh``,
+``Python/template_cam.cu``, ``Python/vector_field.py``).
+
+Sampler contract (the reference samples through CUDA texture objects; the
+build defines their semantics explicitly, identically here and in the HIP
+kernels):
+
+* coordinates are texel coordinates with pixel centres on integers; a
+  normalised coordinate ``(X + off + 0.5) / W_out`` of the reference becomes
+  ``(X + off + 0.5) * W_src / W_out - 0.5`` (``Python/vfs.cu:40-47,56-57``);
+* linear filtering is exact bilinear interpolation (the 8-bit fractional
+  weights of the texture unit are not reproduced);
+* mirror addressing is half-sample symmetric reflection with period 2N
+  (index -1 -> 0, N -> N-1);
+* point filtering takes texel ``floor(xn * W_src)``; the mask pyramid uses the
+  exact integer form ``((2X+1) * Ws) // (2 * Wd)``;
+* stencil neighbours outside the image are read through the same mirror
+  (index -1 -> 0, W -> W-1) for masks and for arrays.  The reference reads
+  arrays at ``index +- 1`` / ``index +- W`` there, which is only defined when
+  the mask's outer 1-px ring is zero; wherever it is defined the two agree.
+
+Parity status of each function is pinned by tests/test_oracle.py (closed
+forms, invariants, brute force); see the test names cited per function.
+"""
+
+import math
+
+import numpy as np
+
+F64 = np.float64
+
+# Hidden constants of the reference (Python/vfs.py:52-54,62-63; Python/vfs.cu:430,518).
+ETA_P = 3.0
+ETA_Q = 2.0
+MIN_NORMALIZED_GRAD = 1e-8
+MIN_TENSOR_VALUE = 1e-8
+EPS = 1e-7
+MEDIAN_KERNEL_SIZE = 5
+
+
+# --------------------------------------------------------------------------------------
+# samplers (Python/vfs.cu:35-143, Python/texture.py:27-55)
+# --------------------------------------------------------------------------------------
+
+def mirror_index(i, n):
+    """Half-sample symmetric reflection with period 2n (cudaAddressModeMirror)."""
+    m = np.mod(i, 2 * n)
+    return np.where(m >= n, 2 * n - 1 - m, m)
+
+
+def sample_linear(img, x, y):
+    """Bilinear sample of ``img`` (H, W[, C]) at texel coordinates (x, y), mirror edges.
+
+    Stands for ``tex2D`` on a Linear/Mirror/normalised texture (Python/vfs.py:96-98).
+    """
+    img = np.asarray(img, dtype=F64)
+    H, W = img.shape[:2]
+    x = np.asarray(x, dtype=F64)
+    y = np.asarray(y, dtype=F64)
+    xc = np.clip(np.nan_to_num(x, nan=0.0), -1e7, 1e7)
+    yc = np.clip(np.nan_to_num(y, nan=0.0), -1e7, 1e7)
+    x0 = np.floor(xc)
+    y0 = np.floor(yc)
+    ax = x - x0
+    ay = y - y0
+    ix0 = mirror_index(x0.astype(np.int64), W)
+    ix1 = mirror_index(x0.astype(np.int64) + 1, W)
+    iy0 = mirror_index(y0.astype(np.int64), H)
+    iy1 = mirror_index(y0.astype(np.int64) + 1, H)
+    t00 = img[iy0, ix0]
+    t10 = img[iy0, ix1]
+    t01 = img[iy1, ix0]
+    t11 = img[iy1, ix1]
+    if img.ndim == 3:
+        ax = ax[..., None]
+        ay = ay[..., None]
+    r0 = t00 + ax * (t10 - t00)
+    r1 = t01 + ax * (t11 - t01)
+    return r0 + ay * (r1 - r0)
+
+
+def texel(img, ix, iy):
+    """Exact texel read with mirror addressing (a Linear fetch at a texel centre)."""
+    H, W = img.shape[:2]
+    return img[mirror_index(iy, H), mirror_index(ix, W)]
+
+
+def grid(H, W):
+    Y, X = np.meshgrid(np.arange(H, dtype=np.int64), np.arange(W, dtype=np.int64), indexing="ij")
+    return X, Y
+
+
+# --------------------------------------------------------------------------------------
+# pyramids (Python/vfs.py:84-218, Python/vfs.cu:145-184)
+# --------------------------------------------------------------------------------------
+
+def compute_pyramid_sizes(height, width, n_level, f_scale):
+    """Python/vfs.py:84-91: sizes[k+1] = (int(h / fScale), int(w / fScale))."""
+    sizes = []
+    h, w = height, width
+    for _ in range(n_level):
+        sizes.append((h, w))
+        h = int(h / f_scale)
+        w = int(w / f_scale)
+    return sizes
+
+
+def cast_and_scale(img_u8):
+    """castAndScale, Python/vfs.cu:145-156: float(u8) / 256."""
+    return np.asarray(img_u8, dtype=np.uint8).astype(F64) / 256.0
+
+
+def downsample_bilinear(src, out_shape):
+    """downSamplingImage / downSamplingVectorField with the Bilinear sampler, Python/vfs.cu:35-49,171-184.
+
+    Four linear taps at +-1/4 output pixel around the output pixel centre.
+    """
+    Hs, Ws = src.shape[:2]
+    Hd, Wd = out_shape
+    X, Y = grid(Hd, Wd)
+    rx = Ws / Wd
+    ry = Hs / Hd
+    xc = (X + 0.5) * rx - 0.5
+    yc = (Y + 0.5) * ry - 0.5
+    xm = (X + 0.25) * rx - 0.5
+    xp = (X + 0.75) * rx - 0.5
+    ym = (Y + 0.25) * ry - 0.5
+    yp = (Y + 0.75) * ry - 0.5
+    return 0.25 * (sample_linear(src, xm, yc) + sample_linear(src, xp, yc)
+                   + sample_linear(src, xc, ym) + sample_linear(src, xc, yp))
+
+
+def downsample_mask(src, out_shape):
+    """downSamplingMask with NearestNeighbor on a Point texture, Python/vfs.cu:54-59,176-179.
+
+    floor((X + 0.5) / Wd * Ws) evaluated exactly as ((2X+1) * Ws) // (2 * Wd).
+    """
+    Hs, Ws = src.shape[:2]
+    Hd, Wd = out_shape
+    X, Y = grid(Hd, Wd)
+    ix = ((2 * X + 1) * Ws) // (2 * Wd)
+    iy = ((2 * Y + 1) * Hs) // (2 * Hd)
+    return np.asarray(src)[iy, ix]
+
+
+def create_mask_pyramid(mask, sizes):
+    """Python/vfs.py:159-187."""
+    masks = [np.asarray(mask, dtype=np.uint8)]
+    for sz in sizes[1:]:
+        masks.append(downsample_mask(masks[-1], sz))
+    return masks
+
+
+def create_vector_field_pyramid(translation, sizes):
+    """Python/vfs.py:189-218 (tvf pyramid only; the calibration field stays at full size)."""
+    tr = [np.asarray(translation, dtype=F64)]
+    for sz in sizes[1:]:
+        tr.append(downsample_bilinear(tr[-1], sz))
+    return tr
+
+
+def create_image_pyramid_(img, sizes):
+    """Python/vfs.py:93-119."""
+    pyr = [img]
+    for sz in sizes[1:]:
+        pyr.append(downsample_bilinear(pyr[-1], sz))
+    return pyr
+
+
+def create_image_pyramid(img_ref, img_other, calibration, sizes):
+    """Python/vfs.py:121-157: cast both, de-rotate the other view by the calibration field, build pyramids."""
+    ref0 = cast_and_scale(img_ref)
+    other0 = cast_and_scale(img_other)
+    refs = create_image_pyramid_(ref0, sizes)
+    other0_warp = warping_image(other0, calibration)
+    others = create_image_pyramid_(other0_warp, sizes)
+    return refs, others
+
+
+# --------------------------------------------------------------------------------------
+# per-level / per-warp kernels
+# --------------------------------------------------------------------------------------
+
+def warping_image(src, warp):
+    """warpingImage, Python/vfs.cu:61-66,186-202: S_lin(src, X + w.x, Y + w.y)."""
+    warp = np.asarray(warp, dtype=F64)
+    H, W = warp.shape[:2]
+    Hs, Ws = src.shape[:2]
+    X, Y = grid(H, W)
+    if (Hs, Ws) == (H, W):
+        xs = X + warp[..., 0]
+        ys = Y + warp[..., 1]
+    else:
+        xs = (X + warp[..., 0] + 0.5) * (Ws / W) - 0.5
+        ys = (Y + warp[..., 1] + 0.5) * (Hs / H) - 0.5
+    return sample_linear(src, xs, ys)
+
+
+def find_warping_vector(warp, tvf):
+    """findWarpingVector, Python/vfs.cu:283-302: S_lin(tvf, X + w.x, Y + w.y)."""
+    return warping_image(tvf, warp)
+
+
+def gaussian_masked(img, mask):
+    """gaussianMasked, Python/vfs.cu:81-102,204-226: binomial 3x3 / 16 with mirror edges; 0 where mask == 0."""
+    img = np.asarray(img, dtype=F64)
+    v0 = img
+    v1 = (_nbr(img, -1, -1) + _nbr(img, -1, 1) + _nbr(img, 1, -1) + _nbr(img, 1, 1))
+    v2 = (_nbr(img, -1, 0) + _nbr(img, 1, 0) + _nbr(img, 0, -1) + _nbr(img, 0, 1))
+    out = 0.25 * v0 + 0.0625 * v1 + 0.125 * v2
+    return np.where(np.asarray(mask) == 0, 0.0, out)
+
+
+def compute_eta(a, b, c, alpha0, alpha1):
+    """computeEta, Python/vfs.cu:228-237."""
+    a11 = alpha1 * alpha1
+    a004 = 4.0 * alpha0 * alpha0
+    e0 = a11 * (a * a + b * b + 2.0 * c * c) + a11 * ((a + c) ** 2 + (b + c) ** 2)
+    e1 = a11 * (b * b + c * c) + a004
+    e2 = a11 * (a * a + c * c) + a004
+    return np.stack([e0, e1, e2], axis=-1)
+
+
+def compute_tensor_eta(img, mask, beta, gamma, alpha0, alpha1):
+    """computeTensorEtaMasked, Python/vfs.cu:104-118,239-281.
+
+    g = (I(X+1,Y+1) - I(X,Y+1), I(X+1,Y+1) - I(X+1,Y)); n = g/|g| (or (1,0) if |g| < 1e-8);
+    T = w n n^T + n_perp n_perp^T with w = max(exp(-beta |g|^gamma), 1e-8); (a, b, c) = (T00, T11, T01).
+    """
+    img = np.asarray(img, dtype=F64)
+    i11 = _nbr(img, 1, 1)
+    gx = i11 - _nbr(img, 0, 1)
+    gy = i11 - _nbr(img, 1, 0)
+    ng = np.hypot(gx, gy)
+    small = ng < MIN_NORMALIZED_GRAD
+    with np.errstate(invalid="ignore", divide="ignore"):
+        nx = np.where(small, 1.0, gx / np.where(small, 1.0, ng))
+        ny = np.where(small, 0.0, gy / np.where(small, 1.0, ng))
+    tx, ty = ny, -nx
+    w = np.maximum(np.exp(-beta * np.power(ng, gamma)), MIN_TENSOR_VALUE)
+    a = w * nx * nx + tx * tx
+    b = w * ny * ny + ty * ty
+    c = w * nx * ny + tx * ty
+    abc = np.stack([a, b, c], axis=-1)
+    eta = compute_eta(a, b, c, alpha0, alpha1)
+    m = (np.asarray(mask) == 0)[..., None]
+    return np.where(m, 0.0, abc), np.where(m, 0.0, eta)
+
+
+def compute_derivatives(warped_other, ref, tvf):
+    """computeDerivatives, Python/vfs.cu:120-142,304-324.
+
+    e = tvf(X,Y)/|tvf(X,Y)|; Iu = [R(x-2e) - 8R(x-e) + 8R(x+e) - R(x+2e) + same on Ow] / 24; Iz = Ow(x) - R(x).
+    """
+    ref = np.asarray(ref, dtype=F64)
+    ow = np.asarray(warped_other, dtype=F64)
+    tvf = np.asarray(tvf, dtype=F64)
+    H, W = ref.shape
+    X, Y = grid(H, W)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        nrm = np.hypot(tvf[..., 0], tvf[..., 1])
+        ex = tvf[..., 0] / nrm
+        ey = tvf[..., 1] / nrm
+
+    def taps(img):
+        v = sample_linear(img, X - 2 * ex, Y - 2 * ey)
+        v = v - 8.0 * sample_linear(img, X - ex, Y - ey)
+        v = v + 8.0 * sample_linear(img, X + ex, Y + ey)
+        v = v - sample_linear(img, X + 2 * ex, Y + 2 * ey)
+        return v
+
+    iu = (taps(ref) + taps(ow)) / 24.0
+    iz = ow - ref
+    return np.stack([iu, iz], axis=-1)
+
+
+# --------------------------------------------------------------------------------------
+# primal-dual iteration (Python/vfs.cu:326-581)
+# --------------------------------------------------------------------------------------
+
+def _nbr(a, dx, dy):
+    """a at (X+dx, Y+dy) for |dx|, |dy| <= 1 with mirror addressing (index -1 -> 0, N -> N-1)."""
+    a = np.asarray(a)
+    out = a
+    if dx == 1:
+        out = np.concatenate([out[:, 1:], out[:, -1:]], axis=1)
+    elif dx == -1:
+        out = np.concatenate([out[:, :1], out[:, :-1]], axis=1)
+    if dy == 1:
+        out = np.concatenate([out[1:], out[-1:]], axis=0)
+    elif dy == -1:
+        out = np.concatenate([out[:1], out[:-1]], axis=0)
+    return out
+
+
+def update_dual_variables(p, q, sigma, u_, v_, abc, mask, alpha0, alpha1):
+    """updateDualVariables, Python/vfs.cu:326-419.  Returns new (p, tp, q); inputs are not modified."""
+    m = np.asarray(mask) != 0
+    mE, mS = _nbr(m, 1, 0), _nbr(m, 0, 1)
+    u_ = np.asarray(u_, dtype=F64)
+    v_ = np.asarray(v_, dtype=F64)
+    uE, uW, uS, uN = _nbr(u_, 1, 0), _nbr(u_, -1, 0), _nbr(u_, 0, 1), _nbr(u_, 0, -1)
+    vE, vW, vS, vN = _nbr(v_, 1, 0), _nbr(v_, -1, 0), _nbr(v_, 0, 1), _nbr(v_, 0, -1)
+    vx, vy = v_[..., 0], v_[..., 1]
+    gux = np.where(mE, uE - u_ - vx, u_ - uW - vx)
+    gvx = np.where(mE, vE[..., 0] - vx, vx - vW[..., 0])
+    gvw = np.where(mE, vE[..., 1] - vy, vy - vW[..., 1])
+    guy = np.where(mS, uS - u_ - vy, u_ - uN - vy)
+    gvy = np.where(mS, vS[..., 1] - vy, vy - vN[..., 1])
+    gvz = np.where(mS, vS[..., 0] - vx, vx - vN[..., 0])
+    a, b, c = abc[..., 0], abc[..., 1], abc[..., 2]
+    tux = a * gux + c * guy
+    tuy = c * gux + b * guy
+    s = alpha1 * sigma / ETA_P
+    px = p[..., 0] + s * tux
+    py = p[..., 1] + s * tuy
+    n = np.maximum(np.hypot(px, py), 1.0)
+    px, py = px / n, py / n
+    tpx = a * px + c * py
+    tpy = c * px + b * py
+    s = alpha0 * sigma / ETA_Q
+    qn = [q[..., 0] + s * gvx, q[..., 1] + s * gvy, q[..., 2] + s * gvz, q[..., 3] + s * gvw]
+    n = np.maximum(np.sqrt(qn[0] ** 2 + qn[1] ** 2 + qn[2] ** 2 + qn[3] ** 2), 1.0)
+    qn = [x / n for x in qn]
+    mm = ~m
+    P = np.stack([px, py], -1)
+    TP = np.stack([tpx, tpy], -1)
+    Q = np.stack(qn, -1)
+    P[mm] = 0.0
+    TP[mm] = 0.0
+    Q[mm] = 0.0
+    return P, TP, Q
+
+
+def l1_thresholding(tau, tp, u, u_, deriv, eta, mask, lam):
+    """l1Thresholding, Python/vfs.cu:421-505: prox of lam|Iu(x-u)+Iz| in the metric eta/tau."""
+    m = np.asarray(mask) != 0
+    mE, mW, mS, mN = _nbr(m, 1, 0), _nbr(m, -1, 0), _nbr(m, 0, 1), _nbr(m, 0, -1)
+    tpx, tpy = tp[..., 0], tp[..., 1]
+    tpWx = _nbr(tp, -1, 0)[..., 0]
+    tpNy = _nbr(tp, 0, -1)[..., 1]
+    gx = np.where(mE & mW, tpx - tpWx, np.where(~mE, -tpWx, tpx))
+    gy = np.where(mS & mN, tpy - tpNy, np.where(~mS, -tpNy, tpy))
+    iu, iz = deriv[..., 0], deriv[..., 1]
+    e0 = eta[..., 0]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        te = np.where(np.abs(e0) < EPS, tau, tau / np.where(e0 == 0, 1.0, e0))
+        ut = u_ + te * (gx + gy)
+        rho = iu * (ut - u) + iz
+        bound = lam * te * iu * iu
+        d0 = ut - u
+        du_in = np.where(np.abs(iu) < EPS, d0, d0 - rho / np.where(iu == 0, 1.0, iu))
+    du = np.where((rho <= bound) & (rho >= -bound), du_in,
+                  np.where(rho < -bound, d0 + lam * te * iu,
+                           np.where(rho > bound, d0 - lam * te * iu, 0.0)))
+    out = u + du
+    return np.where(m, out, 0.0)
+
+
+def update_primal_variables(tau, mu, p, q, u, u_, v_, abc, eta, mask, alpha0, alpha1):
+    """updatePrimalVariables, Python/vfs.cu:507-581.  Returns (u_bar_new, v_bar_new)."""
+    m = np.asarray(mask) != 0
+    mE, mW, mS, mN = _nbr(m, 1, 0), _nbr(m, -1, 0), _nbr(m, 0, 1), _nbr(m, 0, -1)
+    qW = _nbr(q, -1, 0)
+    qN = _nbr(q, 0, -1)
+    gqx = np.where(mE, q[..., 0], 0.0) - np.where(mW, qW[..., 0], 0.0)
+    gqw = np.where(mE, q[..., 3], 0.0) - np.where(mW, qW[..., 3], 0.0)
+    gqy = np.where(mS, q[..., 1], 0.0) - np.where(mN, qN[..., 1], 0.0)
+    gqz = np.where(mS, q[..., 2], 0.0) - np.where(mN, qN[..., 2], 0.0)
+    a, b, c = abc[..., 0], abc[..., 1], abc[..., 2]
+    tux = a * p[..., 0] + c * p[..., 1]
+    tuy = c * p[..., 0] + b * p[..., 1]
+    e1, e2 = eta[..., 1], eta[..., 2]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        t1 = np.where(np.abs(e1) < EPS, tau, tau / np.where(e1 == 0, 1.0, e1))
+        t2 = np.where(np.abs(e2) < EPS, tau, tau / np.where(e2 == 0, 1.0, e2))
+    vx = t1 * (alpha1 * tux + alpha0 * (gqx + gqz)) + v_[..., 0]
+    vy = t2 * (alpha1 * tuy + alpha0 * (gqw + gqy)) + v_[..., 1]
+    un = mu * (u - u_) + u
+    vxn = mu * (vx - v_[..., 0]) + vx
+    vyn = mu * (vy - v_[..., 1]) + vy
+    V = np.stack([vxn, vyn], -1)
+    un = np.where(m, un, 0.0)
+    V[~m] = 0.0
+    return un, V
+
+
+def median_filter(u, mask, ksize=MEDIAN_KERNEL_SIZE):
+    """medianFilter, Python/vfs.cu:583-639: k x k median with clamp-to-edge indices; masked pixels copy."""
+    u = np.asarray(u, dtype=F64)
+    H, W = u.shape
+    h = ksize // 2
+    pad = np.pad(u, h, mode="edge")
+    win = np.lib.stride_tricks.sliding_window_view(pad, (ksize, ksize)).reshape(H, W, ksize * ksize)
+    med = np.partition(win, (ksize * ksize) // 2, axis=-1)[..., (ksize * ksize) // 2]
+    return np.where(np.asarray(mask) == 0, u, med)
+
+
+def compute_optical_flow_with_clamp(flow, u0, uf, tv, mask, limit_range):
+    """computeOpticalFlowWithClamp, Python/vfs.cu:641-676.  Returns new (flow, u0)."""
+    m = np.asarray(mask) != 0
+    delta = np.minimum(uf - u0, limit_range)
+    delta = np.where(m, delta, 0.0)
+    u_new = np.where(m, u0 + delta, u0)
+    f_new = flow + np.where(m[..., None], delta[..., None] * tv, 0.0)
+    return f_new, u_new
+
+
+def upsampling(src, out_shape, f_scale):
+    """upSamplingVectorField(1), Python/vfs.cu:69-79,678-705: fScale * S_lin(src, (X+0.5) Ws/Wd - 0.5)."""
+    Hs, Ws = src.shape[:2]
+    Hd, Wd = out_shape
+    X, Y = grid(Hd, Wd)
+    xs = (X + 0.5) * (Ws / Wd) - 0.5
+    ys = (Y + 0.5) * (Hs / Hd) - 0.5
+    return f_scale * sample_linear(src, xs, ys)
+
+
+def triangulate_rays(rays_ref, rays_other, t, mask):
+    """triangulateRays, Python/vfs.cu:707-745: Z = (t.x - t.z x') / (x' - x); masked -> (0, 0, -1)."""
+    rr = np.asarray(rays_ref, dtype=F64)
+    ro = np.asarray(rays_other, dtype=F64)
+    t = np.asarray(t, dtype=F64).reshape(-1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        xr = rr[..., 0] / rr[..., 2]
+        yr = rr[..., 1] / rr[..., 2]
+        xo = ro[..., 0] / ro[..., 2]
+        z = (t[0] - t[2] * xo) / (xo - xr)
+    out = np.stack([z * xr, z * yr, z], -1)
+    out[np.asarray(mask) == 0] = (0.0, 0.0, -1.0)
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# camera (Python/equidistant_cam.cuh:32-59, Python/template_cam.cu:25-86)
+# --------------------------------------------------------------------------------------
+
+def project(xyz, f, u0, v0):
+    """EquidistantCamera::project: theta = atan2(|xy|, z), phi = atan2(y, x), uv = f theta (cos, sin) + c."""
+    xyz = np.asarray(xyz, dtype=F64)
+    r = np.hypot(xyz[..., 0], xyz[..., 1])
+    th = np.arctan2(r, xyz[..., 2])
+    ph = np.arctan2(xyz[..., 1], xyz[..., 0])
+    return np.stack([f * th * np.cos(ph) + u0, f * th * np.sin(ph) + v0], -1)
+
+
+def unproject(uv, f, u0, v0):
+    """EquidistantCamera::unproject: theta = |uv - c| / f, X = (sin th cos ph, sin th sin ph, cos th)."""
+    uv = np.asarray(uv, dtype=F64)
+    x = uv[..., 0] - u0
+    y = uv[..., 1] - v0
+    th = np.hypot(x, y) / f
+    ph = np.arctan2(y, x)
+    return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+
+
+def get_rays(shape, f, u0, v0):
+    """getRays, Python/template_cam.cu:25-37: unproject integer pixel coordinates (no +0.5)."""
+    H, W = shape
+    X, Y = grid(H, W)
+    return unproject(np.stack([X, Y], -1).astype(F64), f, u0, v0)
+
+
+def get_grid_zero(shape):
+    """Python/vector_field.py:58-62."""
+    H, W = shape
+    X, Y = grid(H, W)
+    return np.stack([X, Y], -1).astype(F64)
+
+
+def get_vector_fields(grid_uv, cam_ref, cam_other, R, t, t_scale=1.0):
+    """Python/vector_field.py:28-56 with the unit-norm normaliser of Python/main_equidistant.py:60.
+
+    cam_* are (f, u0, v0).  Returns (tvf, cvf) in float64.
+    """
+    if isinstance(t_scale, (int, float)):
+        t_scale = (t_scale, t_scale)
+    t = np.asarray(t, dtype=F64).reshape(3)
+    xyz = unproject(grid_uv, *cam_ref)
+    xyz = xyz / np.linalg.norm(xyz, axis=2)[..., None]
+    fwd = project(xyz + t_scale[0] * t, *cam_ref)
+    bwd = project(xyz - t_scale[1] * t, *cam_ref)
+    tvf = (fwd - bwd) / 2.0
+    tvf = tvf / np.linalg.norm(tvf, axis=2)[..., None]
+    rxyz = xyz @ np.asarray(R, dtype=F64).T
+    cvf = project(rxyz, *cam_other) - grid_uv
+    return tvf, cvf
+
+
+# --------------------------------------------------------------------------------------
+# host loop (Python/vfs.py:460-559)
+# --------------------------------------------------------------------------------------
+
+def pd_schedule(n_iter, time_step_lambda):
+    """The tau/sigma/mu sequence of Python/vfs.py:494-515, evaluated with NumPy >= 2 scalar rules.
+
+    tau starts as a Python float; mu is float32 (``.astype(np.float32)``), and every later
+    product with it stays float32 because Python scalars are weak.  Returns float64 arrays
+    holding those float32 values, one entry per iteration (the values passed to the kernels).
+    """
+    taus, sigmas, mus = [], [], []
+    tau = 1.0
+    sigma = 1.0 / tau
+    for _ in range(n_iter):
+        if sigma < 1000:
+            mu = 1 / np.sqrt(1 + 0.7 * tau * time_step_lambda).astype(np.float32)
+        else:
+            mu = 1
+        taus.append(float(np.float32(tau)))
+        sigmas.append(float(np.float32(sigma)))
+        mus.append(float(np.float32(mu)))
+        sigma = sigma / mu
+        tau = tau * mu
+    return np.array(taus), np.array(sigmas), np.array(mus)
+
+
+class OracleParams:
+    """Mirror of Python/vfs_params.py:25-43 (attribute names kept)."""
+
+    def __init__(self, height, width, **kw):
+        self.width = width
+        self.height = height
+        self.beta = 9.0
+        self.gamma = 0.85
+        self.alpha0 = 17.0
+        self.alpha1 = 1.2
+        self.timeStepLambda = 1.0
+        self.Lambda = 5.0
+        self.nLevel = 11
+        self.fScale = 1.2
+        self.nWarpIters = 100
+        self.nSolverIters = 100
+        self.limitRange = 0.1
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+def solve_stereo(img_ref, img_other, mask, tvf, cvf, prm, trace=None):
+    """VFS.solve_stereo, Python/vfs.py:460-533 (with the pyramids of Python/vfs.py:159-218).
+
+    ``trace``, if given, is a dict that receives the per-level intermediate state
+    (used by tests to compare the GPU path stage by stage).
+    """
+    sizes = compute_pyramid_sizes(prm.height, prm.width, prm.nLevel, prm.fScale)
+    masks = create_mask_pyramid(mask, sizes)
+    trs = create_vector_field_pyramid(tvf, sizes)
+    refs, others = create_image_pyramid(img_ref, img_other, np.asarray(cvf, F64), sizes)
+    taus, sigmas, mus = pd_schedule(prm.nSolverIters, prm.timeStepLambda)
+    warp = u = u_ = None
+    for k in range(len(sizes), 0, -1):
+        s = sizes[k - 1]
+        if k == len(sizes):
+            warp = np.zeros((*s, 2))
+            u = np.zeros(s)
+            u_ = np.zeros(s)
+        msk = masks[k - 1]
+        ref_s = gaussian_masked(refs[k - 1], msk)
+        abc, eta = compute_tensor_eta(ref_s, msk, prm.beta, prm.gamma, prm.alpha0, prm.alpha1)
+        for kw in range(prm.nWarpIters):
+            tv = find_warping_vector(warp, trs[k - 1])
+            ow = warping_image(others[k - 1], warp)
+            deriv = compute_derivatives(ow, refs[k - 1], trs[k - 1])
+            p = np.zeros((*s, 2))
+            q = np.zeros((*s, 4))
+            v_ = np.zeros((*s, 2))
+            u0 = u.copy()
+            for ks in range(prm.nSolverIters):
+                p, tp, q = update_dual_variables(p, q, sigmas[ks], u_, v_, abc, msk, prm.alpha0, prm.alpha1)
+                u = l1_thresholding(taus[ks], tp, u, u_, deriv, eta, msk, prm.Lambda)
+                u_, v_ = update_primal_variables(taus[ks], mus[ks], p, q, u, u_, v_, abc, eta, msk,
+                                                 prm.alpha0, prm.alpha1)
+            uf = median_filter(u, msk)
+            warp, u = compute_optical_flow_with_clamp(warp, u0, uf, tv, msk, prm.limitRange)
+        if trace is not None:
+            trace[k - 1] = dict(warp=warp.copy(), u=u.copy(), u_=u_.copy())
+        if k > 1:
+            ss = sizes[k - 2]
+            u_ = upsampling(u, ss, prm.fScale)
+            u = u_.copy()
+            warp = upsampling(warp, ss, prm.fScale)
+    warp = warp.copy()
+    warp[masks[0] == 0, :] = 0.0
+    return warp
+
+
+def triangulate_flow(flow, cam, t, mask):
+    """VFS.triangulate_flow, Python/vfs.py:535-559."""
+    rays = get_rays(flow.shape[:2], *cam)
+    uv = get_grid_zero(flow.shape[:2])
+    rays_other = unproject(uv + flow, *cam)
+    return triangulate_rays(rays, rays_other, t, mask)

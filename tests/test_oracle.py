@@ -1,0 +1,399 @@
+"""Pins of the float64 oracle (oracle/vfs_oracle.py) by closed forms, library routines,
+invariants and brute force -- independent of the oracle's own formulas.
+
+The reference ships no golden vectors for the solver (its only test compares
+vector fields with external dataset files, Python/test_vector_field.py:52-66),
+so these checks carry the parity of the oracle itself.
+"""
+
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy import ndimage, optimize
+
+from oracle import vfs_oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+# ---- samplers -----------------------------------------------------------------------
+
+def test_mirror_index_is_half_sample_symmetric():
+    n = 5
+    idx = np.arange(-12, 13)
+    padded = np.pad(np.arange(n), 15, mode="symmetric")  # ... 1 0 | 0 1 2 3 4 | 4 3 ...
+    np.testing.assert_array_equal(O.mirror_index(idx, n), padded[idx + 15])
+
+
+def test_sample_linear_matches_scipy_map_coordinates():
+    img = RNG.random((13, 17))
+    x = RNG.uniform(-4, 20, 500)
+    y = RNG.uniform(-4, 16, 500)
+    # scipy 'reflect' is half-sample symmetric (d c b a | a b c d | d c b a)
+    ref = ndimage.map_coordinates(img, [y, x], order=1, mode="reflect")
+    np.testing.assert_allclose(O.sample_linear(img, x, y), ref, rtol=0, atol=1e-12)
+
+
+def test_sample_linear_integer_shift_and_linearity():
+    img = RNG.random((9, 11))
+    X, Y = O.grid(9, 11)
+    pad = np.pad(img, 3, mode="symmetric")
+    np.testing.assert_array_equal(O.sample_linear(img, X + 2, Y - 1), pad[3 - 1:3 - 1 + 9, 3 + 2:3 + 2 + 11])
+    a, b = RNG.random((9, 11)), RNG.random((9, 11))
+    xs, ys = X + RNG.uniform(-2, 2, X.shape), Y + RNG.uniform(-2, 2, Y.shape)
+    np.testing.assert_allclose(O.sample_linear(2 * a - 3 * b, xs, ys),
+                               2 * O.sample_linear(a, xs, ys) - 3 * O.sample_linear(b, xs, ys), atol=1e-12)
+
+
+def test_sample_linear_nan_coordinate_gives_nan():
+    img = RNG.random((4, 4))
+    assert np.isnan(O.sample_linear(img, np.array([np.nan]), np.array([1.0]))[0])
+
+
+# ---- pyramids -----------------------------------------------------------------------
+
+def test_pyramid_sizes_truncate():
+    assert O.compute_pyramid_sizes(800, 848, 4, 1.2) == [(800, 848), (666, 706), (555, 588), (462, 490)]
+    assert O.compute_pyramid_sizes(800, 800, 5, 2.0)[-1] == (50, 50)
+
+
+def test_cast_and_scale_divides_by_256():
+    u8 = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    np.testing.assert_array_equal(O.cast_and_scale(u8), np.arange(256).reshape(16, 16) / 256.0)
+
+
+def test_downsample_ratio2_is_box_mean():
+    img = RNG.random((32, 48))
+    box = img.reshape(16, 2, 24, 2).mean(axis=(1, 3))
+    np.testing.assert_allclose(O.downsample_bilinear(img, (16, 24)), box, atol=1e-14)
+    vf = RNG.random((32, 48, 2))
+    np.testing.assert_allclose(O.downsample_bilinear(vf, (16, 24)),
+                               vf.reshape(16, 2, 24, 2, 2).mean(axis=(1, 3)), atol=1e-14)
+
+
+@pytest.mark.parametrize("hs,ws,hd,wd", [(64, 64, 32, 32), (25, 37, 12, 18), (800, 848, 666, 706), (11, 7, 9, 5)])
+def test_mask_index_rule_is_exact_floor(hs, ws, hd, wd):
+    src = (RNG.random((hs, ws)) > 0.5).astype(np.uint8) * 255
+    out = O.downsample_mask(src, (hd, wd))
+    ix = [int((Fraction(2 * x + 1, 2 * wd) * ws) // 1) for x in range(wd)]
+    iy = [int((Fraction(2 * y + 1, 2 * hd) * hs) // 1) for y in range(hd)]
+    np.testing.assert_array_equal(out, src[np.ix_(iy, ix)])
+    if hs == 2 * hd and ws == 2 * wd:
+        np.testing.assert_array_equal(out, src[1::2, 1::2])
+
+
+# ---- per-level / per-warp kernels ---------------------------------------------------------
+
+def test_warp_zero_flow_identity_and_integer_flow_shift():
+    img = RNG.random((10, 12))
+    np.testing.assert_array_equal(O.warping_image(img, np.zeros((10, 12, 2))), img)
+    w = np.zeros((10, 12, 2))
+    w[..., 0], w[..., 1] = -3, 2
+    pad = np.pad(img, 4, mode="symmetric")
+    np.testing.assert_array_equal(O.warping_image(img, w), pad[4 + 2:4 + 12, 4 - 3:4 - 3 + 12])
+
+
+def test_gaussian_matches_scipy_binomial_convolution():
+    img = RNG.random((15, 19))
+    mask = (RNG.random((15, 19)) > 0.3).astype(np.uint8)
+    k = np.outer([1, 2, 1], [1, 2, 1]) / 16.0
+    ref = ndimage.convolve(img, k, mode="reflect")
+    np.testing.assert_allclose(O.gaussian_masked(img, mask), np.where(mask > 0, ref, 0.0), atol=1e-14)
+
+
+def _interior(a, m=3):
+    return a[m:-m, m:-m]
+
+
+def test_tensor_eigen_structure_and_eta_matrix_form():
+    img = RNG.random((20, 24)) * 0.5
+    mask = np.ones((20, 24), np.uint8)
+    mask[0, 0] = 0
+    beta, gamma, a0, a1 = 9.0, 0.85, 17.0, 1.2
+    abc, eta = O.compute_tensor_eta(img, mask, beta, gamma, a0, a1)
+    assert np.all(abc[0, 0] == 0) and np.all(eta[0, 0] == 0)
+    a, b, c = abc[..., 0], abc[..., 1], abc[..., 2]
+    T = np.stack([np.stack([a, c], -1), np.stack([c, b], -1)], -2)
+    # g from forward differences anchored at (X+1, Y+1) (edge-replicated)
+    pad = np.pad(img, ((0, 1), (0, 1)), mode="edge")
+    gx = pad[1:, 1:] - pad[1:, :-1]
+    gy = pad[1:, 1:] - pad[:-1, 1:]
+    ng = np.hypot(gx, gy)
+    w = np.maximum(np.exp(-beta * ng ** gamma), 1e-8)
+    ev = np.linalg.eigvalsh(T)
+    sel = mask > 0
+    np.testing.assert_allclose(np.sort(ev, -1)[sel], np.sort(np.stack([w, np.ones_like(w)], -1), -1)[sel], atol=1e-12)
+    n = np.stack([gx, gy], -1) / ng[..., None]
+    Tn = np.einsum("...ij,...j->...i", T, n)
+    np.testing.assert_allclose(Tn[sel], (w[..., None] * n)[sel], atol=1e-12)
+    # eta in matrix form: |T|_F^2 + |T 1|^2, |T e_y|^2 + 4 a0^2 / a1^2, |T e_x|^2 + ...
+    ones = np.einsum("...ij,j->...i", T, np.ones(2))
+    tey = np.einsum("...ij,j->...i", T, np.array([0.0, 1.0]))
+    tex = np.einsum("...ij,j->...i", T, np.array([1.0, 0.0]))
+    e0 = a1 ** 2 * (np.sum(T ** 2, axis=(-1, -2)) + np.sum(ones ** 2, -1))
+    e1 = a1 ** 2 * np.sum(tey ** 2, -1) + 4 * a0 ** 2
+    e2 = a1 ** 2 * np.sum(tex ** 2, -1) + 4 * a0 ** 2
+    np.testing.assert_allclose(eta[sel], np.stack([e0, e1, e2], -1)[sel], rtol=1e-12)
+
+
+def test_tensor_flat_image_is_identity_with_default_normal():
+    abc, _ = O.compute_tensor_eta(np.full((6, 7), 0.3), np.ones((6, 7), np.uint8), 9.0, 0.85, 17.0, 1.2)
+    # g = 0 -> n = (1, 0), w = exp(0) = 1 -> T = I
+    np.testing.assert_allclose(abc, np.broadcast_to([1.0, 1.0, 0.0], abc.shape))
+
+
+def test_derivatives_of_linear_ramp_are_directional_slope():
+    Hh, Ww = 30, 40
+    X, Y = O.grid(Hh, Ww)
+    al, be = 0.013, -0.007
+    ref = al * X + be * Y + 0.2
+    ow = ref + 0.05
+    ang = RNG.uniform(0, 2 * np.pi, (Hh, Ww))
+    tvf = np.stack([np.cos(ang), np.sin(ang)], -1) * RNG.uniform(0.5, 2.0, (Hh, Ww, 1))
+    d = O.compute_derivatives(ow, ref, tvf)
+    slope = al * np.cos(ang) + be * np.sin(ang)
+    np.testing.assert_allclose(_interior(d[..., 0]), _interior(slope), atol=1e-12)
+    np.testing.assert_allclose(d[..., 1], 0.05, atol=1e-12)
+
+
+# ---- primal-dual ---------------------------------------------------------------------
+
+def _params():
+    return dict(alpha0=5.0, alpha1=1.0)
+
+
+def test_dual_step_on_ramp_is_unprojected_closed_form():
+    Hh, Ww = 12, 14
+    X, Y = O.grid(Hh, Ww)
+    u_ = 1e-3 * (3 * X - 2 * Y)
+    v_ = np.zeros((Hh, Ww, 2))
+    v_[..., 0] = 1e-3 * X
+    abc = np.zeros((Hh, Ww, 3))
+    abc[..., 0], abc[..., 1], abc[..., 2] = 0.8, 0.6, 0.1
+    mask = np.ones((Hh, Ww), np.uint8)
+    sigma = 1.3
+    p, tp, q = O.update_dual_variables(np.zeros((Hh, Ww, 2)), np.zeros((Hh, Ww, 4)), sigma, u_, v_, abc, mask, **_params())
+    g = np.array([3e-3 - 1e-3 * X[5, 5], -2e-3])  # grad u - v at pixel (5,5)
+    T = np.array([[0.8, 0.1], [0.1, 0.6]])
+    pe = 1.0 * sigma / 3.0 * (T @ g)
+    np.testing.assert_allclose(p[5, 5], pe, rtol=1e-12)
+    np.testing.assert_allclose(tp[5, 5], T @ pe, rtol=1e-12)
+    # grad v: d/dx v1 = 1e-3 -> q.x; everything else 0
+    np.testing.assert_allclose(q[5, 5], 5.0 * sigma / 2.0 * np.array([1e-3, 0, 0, 0]), rtol=1e-12)
+
+
+def test_dual_projection_and_mask():
+    Hh, Ww = 10, 10
+    mask = (RNG.random((Hh, Ww)) > 0.2).astype(np.uint8)
+    abc = np.broadcast_to([1.0, 1.0, 0.0], (Hh, Ww, 3)).copy()
+    p, tp, q = O.update_dual_variables(RNG.normal(size=(Hh, Ww, 2)), RNG.normal(size=(Hh, Ww, 4)), 50.0,
+                                       RNG.normal(size=(Hh, Ww)), RNG.normal(size=(Hh, Ww, 2)), abc, mask, **_params())
+    assert np.all(np.linalg.norm(p, axis=-1) <= 1 + 1e-12)
+    assert np.all(np.linalg.norm(q, axis=-1) <= 1 + 1e-12)
+    assert np.all(p[mask == 0] == 0) and np.all(q[mask == 0] == 0) and np.all(tp[mask == 0] == 0)
+
+
+def test_dual_uses_backward_difference_where_east_is_masked():
+    Hh, Ww = 6, 8
+    X, Y = O.grid(Hh, Ww)
+    u_ = 1e-3 * X.astype(float) ** 2
+    mask = np.ones((Hh, Ww), np.uint8)
+    mask[:, 5] = 0  # pixel (2, 4) has its east neighbour masked
+    abc = np.broadcast_to([1.0, 1.0, 0.0], (Hh, Ww, 3)).copy()
+    p, _, _ = O.update_dual_variables(np.zeros((Hh, Ww, 2)), np.zeros((Hh, Ww, 4)), 1.0, u_, np.zeros((Hh, Ww, 2)),
+                                      abc, mask, **_params())
+    s = 1.0 / 3.0
+    np.testing.assert_allclose(p[2, 4, 0], s * 1e-3 * (16 - 9))   # backward: u(4) - u(3)
+    np.testing.assert_allclose(p[2, 3, 0], s * 1e-3 * (16 - 9))   # forward: u(4) - u(3)
+    np.testing.assert_allclose(p[2, 2, 0], s * 1e-3 * (9 - 4))
+
+
+def test_threshold_is_prox_of_weighted_l1_by_brute_force():
+    lam = 5.0
+    for _ in range(60):
+        tau = RNG.uniform(0.1, 1.0)
+        eta0 = RNG.uniform(0.5, 5.0)
+        iu = RNG.normal() * RNG.choice([1e-3, 0.1, 1.0])
+        iz = RNG.normal() * 0.1
+        u = RNG.normal() * 0.5
+        u_bar = RNG.normal() * 0.5
+        one = np.ones((3, 3))
+        mask = np.ones((3, 3), np.uint8)
+        tp = np.zeros((3, 3, 2))
+        out = O.l1_thresholding(tau, tp, u * one, u_bar * one, np.stack([iu * one, iz * one], -1),
+                                np.stack([eta0 * one] * 3, -1), mask, lam)[1, 1]
+        te = tau / eta0
+        f = lambda x: (x - u_bar) ** 2 / (2 * te) + lam * abs(iu * (x - u) + iz)  # noqa: E731
+        kink = u - iz / iu
+        cands = [optimize.minimize_scalar(f, bounds=(lo, hi), method="bounded", options=dict(xatol=1e-12)).x
+                 for lo, hi in ((min(kink, u_bar) - 50, kink), (kink, max(kink, u_bar) + 50))] + [kink]
+        best = min(cands, key=f)
+        assert f(out) <= f(best) + 1e-9, (out, best)
+
+
+def test_threshold_divergence_cases():
+    # tp nonzero only at (2,2): divergence seen from (2,3) is -tp(2,2).x when E is out, etc.
+    mask = np.ones((5, 5), np.uint8)
+    mask[2, 4] = 0
+    tp = np.zeros((5, 5, 2))
+    tp[2, 3, 0] = 0.5
+    tp[2, 2, 0] = 0.25
+    zeros = np.zeros((5, 5))
+    eta = np.ones((5, 5, 3))
+    deriv = np.zeros((5, 5, 2))  # Iu = 0, Iz = 0 -> rho = 0 <= bound = 0: du = ut - u
+    out = O.l1_thresholding(1.0, tp, zeros, zeros, deriv, eta, mask, 5.0)
+    assert out[2, 3] == pytest.approx(-0.25)          # E masked: -tp_W.x
+    assert out[2, 2] == pytest.approx(0.25 - 0.0)     # interior: tp.x - tp_W.x
+
+
+def test_primal_linear_q_divergence_and_mu_zero():
+    Hh, Ww = 8, 9
+    X, Y = O.grid(Hh, Ww)
+    q = np.zeros((Hh, Ww, 4))
+    q[..., 0] = 0.01 * X          # d/dx q.x = 0.01 (backward)
+    q[..., 1] = -0.02 * Y         # d/dy q.y = -0.02
+    p = np.zeros((Hh, Ww, 2))
+    p[..., 0] = 0.3
+    abc = np.broadcast_to([0.9, 0.7, 0.2], (Hh, Ww, 3)).copy()
+    eta = np.broadcast_to([3.0, 2.0, 4.0], (Hh, Ww, 3)).copy()
+    mask = np.ones((Hh, Ww), np.uint8)
+    u = RNG.normal(size=(Hh, Ww))
+    u_ = RNG.normal(size=(Hh, Ww))
+    v_ = RNG.normal(size=(Hh, Ww, 2))
+    tau = 0.7
+    un, vn = O.update_primal_variables(tau, 0.0, p, q, u, u_, v_, abc, eta, mask, **_params())
+    Tp = np.array([[0.9, 0.2], [0.2, 0.7]]) @ np.array([0.3, 0.0])
+    ex = v_[4, 4] + np.array([tau / 2.0, tau / 4.0]) * (1.0 * Tp + 5.0 * np.array([0.01, -0.02]))
+    np.testing.assert_allclose(vn[4, 4], ex, rtol=1e-12)
+    np.testing.assert_allclose(un, u, rtol=0, atol=0)
+    un2, _ = O.update_primal_variables(tau, 0.5, p, q, u, u_, v_, abc, eta, mask, **_params())
+    np.testing.assert_allclose(un2, u + 0.5 * (u - u_), atol=1e-12)
+
+
+def test_median_matches_scipy_median_filter():
+    u = RNG.normal(size=(17, 13))
+    mask = (RNG.random((17, 13)) > 0.3).astype(np.uint8)
+    ref = ndimage.median_filter(u, size=5, mode="nearest")
+    np.testing.assert_array_equal(O.median_filter(u, mask), np.where(mask > 0, ref, u))
+    ref3 = ndimage.median_filter(u, size=3, mode="nearest")
+    np.testing.assert_array_equal(O.median_filter(u, mask, 3), np.where(mask > 0, ref3, u))
+
+
+def test_flow_clamp_invariants():
+    Hh, Ww = 7, 8
+    mask = (RNG.random((Hh, Ww)) > 0.3).astype(np.uint8)
+    flow = RNG.normal(size=(Hh, Ww, 2))
+    u0 = RNG.normal(size=(Hh, Ww))
+    uf = u0 + RNG.normal(size=(Hh, Ww)) * 0.3
+    tv = RNG.normal(size=(Hh, Ww, 2))
+    f2, u2 = O.compute_optical_flow_with_clamp(flow, u0, uf, tv, mask, 0.1)
+    d = u2 - u0
+    assert np.all(d <= 0.1 + 1e-15)
+    np.testing.assert_allclose(f2 - flow, d[..., None] * tv, atol=1e-14)
+    m = mask == 0
+    assert np.all(d[m] == 0) and np.all(f2[m] == flow[m])
+    np.testing.assert_allclose(u2[~m], np.minimum(uf, u0 + 0.1)[~m], atol=1e-14)
+
+
+def test_upsample_ratio2_separable_weights():
+    src = RNG.random((6, 7))
+    out = O.upsampling(src, (12, 14), 2.0)
+
+    def up1(a, axis):
+        a = np.moveaxis(a, axis, 0)
+        pad = np.concatenate([a[:1], a, a[-1:]], 0)
+        even = 0.75 * pad[1:-1] + 0.25 * pad[:-2]
+        odd = 0.75 * pad[1:-1] + 0.25 * pad[2:]
+        res = np.empty((2 * a.shape[0],) + a.shape[1:])
+        res[0::2], res[1::2] = even, odd
+        return np.moveaxis(res, 0, axis)
+
+    np.testing.assert_allclose(out, 2.0 * up1(up1(src, 0), 1), atol=1e-14)
+
+
+# ---- camera, vector field, triangulation ----------------------------------------------
+
+CAM = (256 / np.pi, 128.0, 128.0)
+
+
+def test_camera_round_trip_and_equidistance():
+    uv = np.stack(np.meshgrid(np.arange(0, 256, 7.0), np.arange(0, 256, 5.0)), -1)
+    xyz = O.unproject(uv, *CAM)
+    np.testing.assert_allclose(np.linalg.norm(xyz, axis=-1), 1.0, atol=1e-14)
+    r = np.hypot(uv[..., 0] - CAM[1], uv[..., 1] - CAM[2])
+    np.testing.assert_allclose(np.arccos(np.clip(xyz[..., 2], -1, 1)), r / CAM[0], atol=1e-7)
+    np.testing.assert_allclose(O.project(xyz * 3.7, *CAM), uv, atol=1e-9)
+    np.testing.assert_allclose(O.unproject(np.array([CAM[1], CAM[2]]), *CAM), [0, 0, 1], atol=1e-15)
+    rim = O.unproject(np.array([CAM[1] + CAM[0] * np.pi / 2, CAM[2]]), *CAM)
+    np.testing.assert_allclose(rim, [1, 0, 0], atol=1e-12)
+
+
+def test_vector_fields_identity_rotation_and_unit_norm():
+    g = O.get_grid_zero((40, 50))
+    cam = (50 / np.pi, 25.0, 20.0)
+    tvf, cvf = O.get_vector_fields(g, cam, cam, np.eye(3), np.array([[0.2], [0.0], [0.0]]), 1)
+    np.testing.assert_allclose(cvf, 0, atol=1e-9)
+    n = np.linalg.norm(tvf, axis=-1)
+    ok = np.isfinite(n)
+    np.testing.assert_allclose(n[ok], 1.0, atol=1e-12)
+
+
+def test_vector_field_is_limit_direction_of_translation_flow():
+    g = O.get_grid_zero((30, 30))
+    cam = (30 / np.pi, 15.0, 15.0)
+    t = np.array([0.3, 0.05, 0.02])
+    tvf, _ = O.get_vector_fields(g, cam, cam, np.eye(3), t * 1e-4, 1)
+    d = O.unproject(g, *cam)
+    flow = O.project(d * 5.0 + t * 1e-4, *cam) - g   # a point at 5 m, tiny baseline
+    fn = flow / np.linalg.norm(flow, axis=-1, keepdims=True)
+    inside = np.hypot(g[..., 0] - 15, g[..., 1] - 15) < 12
+    cosang = np.sum(fn * tvf, -1)[inside]
+    assert np.all(cosang > 1 - 1e-6)
+
+
+def test_triangulation_is_exact_on_true_flow():
+    Hh, Ww = 40, 40
+    cam = (Ww / np.pi, Ww / 2, Hh / 2)
+    g = O.get_grid_zero((Hh, Ww))
+    d = O.unproject(g, *cam)
+    D = 2.0 + RNG.random((Hh, Ww)) * 3
+    t = np.array([0.25, 0.0, 0.03])
+    P = D[..., None] * d
+    flow = O.project(P + t, *cam) - g
+    mask = (np.hypot(g[..., 0] - Ww / 2, g[..., 1] - Hh / 2) < 15).astype(np.uint8)
+    xyz = O.triangulate_flow(flow, cam, t, mask)
+    np.testing.assert_allclose(xyz[mask > 0], P[mask > 0], rtol=1e-9)
+    assert np.all(xyz[mask == 0] == [0, 0, -1])
+
+
+# ---- host loop --------------------------------------------------------------------------
+
+def test_schedule_closed_form():
+    taus, sigmas, mus = O.pd_schedule(100, 1.0)
+    t = s = 1.0
+    for k in range(100):
+        m = 1.0 / np.sqrt(1 + 0.7 * t)
+        assert taus[k] == pytest.approx(t, rel=2e-6)
+        assert sigmas[k] == pytest.approx(s, rel=2e-6)
+        assert mus[k] == pytest.approx(m, rel=2e-6)
+        t, s = t * m, s / m
+    np.testing.assert_allclose(taus * sigmas, 1.0, rtol=1e-5)
+    assert mus[0] == np.float32(1 / np.float32(np.sqrt(1.7)))
+
+
+def test_oracle_solve_recovers_known_flow(small_case):
+    """End-to-end sanity on the seeded 256x256 pair: the flow approaches the true flow."""
+    c = small_case
+    pair, prm = c["pair"], c["prm"]
+    flow = O.solve_stereo(pair["img_ref"], pair["img_other"], pair["mask"], c["tvf"], c["cvf"], prm)
+    Hh, Ww = flow.shape[:2]
+    cam = pair["cam"]
+    g = O.get_grid_zero((Hh, Ww))
+    P = pair["range_gt"][..., None] * O.unproject(g, *cam)
+    gt = O.project(P + pair["t"].reshape(3), *cam) - g
+    th = np.hypot(g[..., 0] - cam[1], g[..., 1] - cam[2]) / cam[0]
+    m = (pair["mask"] > 0) & (th < np.radians(75))
+    err = np.linalg.norm(flow - gt, axis=-1)[m]
+    assert np.median(err) < 0.6
+    assert np.median(np.linalg.norm(gt, axis=-1)[m]) > 3.0

@@ -1,0 +1,126 @@
+// Device-side helpers shared by the VFS kernels (gfx950).
+//
+// Sampling semantics (stand-in for the reference's CUDA texture objects,
+// Python/texture.py:27-55 and the descriptors at Python/vfs.py:96-98,169-171):
+//   * texel coordinates with pixel centres on integers;
+//   * bilinear interpolation with exact fp32 weights;
+//   * half-sample symmetric mirror addressing with period 2N.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vfs_hip.h"
+
+#define VFS_BLOCK_X 64
+#define VFS_BLOCK_Y 4
+
+namespace vfs {
+
+// Half-sample symmetric reflection (cudaAddressModeMirror): -1 -> 0, n -> n-1.
+__device__ __forceinline__ int mirror(int i, int n)
+{
+    if (static_cast<unsigned>(i) < static_cast<unsigned>(n)) return i;
+    const int p = 2 * n;
+    int m = i % p;
+    if (m < 0) m += p;
+    return m >= n ? p - 1 - m : m;
+}
+
+// Neighbour index for +-1 offsets (mirror of -1 is 0, of n is n-1).
+__device__ __forceinline__ int clamp_idx(int i, int n)
+{
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+struct Tap {
+    int i00, i10, i01, i11;  // element offsets (pixel index, not scaled by channels)
+    float ax, ay;
+};
+
+// Bilinear tap set at texel coordinate (x, y).  Non-finite coordinates keep
+// their NaN/inf in the weights so that the sample is non-finite, while the
+// indices stay in range.
+__device__ __forceinline__ Tap make_tap(float x, float y, int H, int W)
+{
+    const float xc = fminf(fmaxf(x, -1.0e7f), 1.0e7f);
+    const float yc = fminf(fmaxf(y, -1.0e7f), 1.0e7f);
+    const float fx = floorf(xc);
+    const float fy = floorf(yc);
+    const int ix = static_cast<int>(fx);
+    const int iy = static_cast<int>(fy);
+    const int x0 = mirror(ix, W), x1 = mirror(ix + 1, W);
+    const int y0 = mirror(iy, H), y1 = mirror(iy + 1, H);
+    Tap t;
+    t.i00 = y0 * W + x0;
+    t.i10 = y0 * W + x1;
+    t.i01 = y1 * W + x0;
+    t.i11 = y1 * W + x1;
+    t.ax = x - fx;
+    t.ay = y - fy;
+    return t;
+}
+
+__device__ __forceinline__ float lerp2(float t00, float t10, float t01, float t11, float ax, float ay)
+{
+    const float r0 = fmaf(ax, t10 - t00, t00);
+    const float r1 = fmaf(ax, t11 - t01, t01);
+    return fmaf(ay, r1 - r0, r0);
+}
+
+__device__ __forceinline__ float sample1(const float* __restrict__ img, int H, int W, float x, float y)
+{
+    const Tap t = make_tap(x, y, H, W);
+    return lerp2(img[t.i00], img[t.i10], img[t.i01], img[t.i11], t.ax, t.ay);
+}
+
+__device__ __forceinline__ float2 sample2(const float2* __restrict__ img, int H, int W, float x, float y)
+{
+    const Tap t = make_tap(x, y, H, W);
+    const float2 a = img[t.i00], b = img[t.i10], c = img[t.i01], d = img[t.i11];
+    return make_float2(lerp2(a.x, b.x, c.x, d.x, t.ax, t.ay), lerp2(a.y, b.y, c.y, d.y, t.ax, t.ay));
+}
+
+// Kernel-side copy of the solver constants.
+struct KParams {
+    float beta, gamma, alpha0, alpha1, eta_p, eta_q, lambda, limit_range;
+    float f_scale, min_grad, min_tensor, a11, a004;
+};
+
+inline KParams to_kparams(const vfs_params* p)
+{
+    KParams k;
+    k.beta = p->beta;
+    k.gamma = p->gamma;
+    k.alpha0 = p->alpha0;
+    k.alpha1 = p->alpha1;
+    k.eta_p = p->eta_p;
+    k.eta_q = p->eta_q;
+    k.lambda = p->lambda;
+    k.limit_range = p->limit_range;
+    k.f_scale = p->f_scale;
+    k.min_grad = p->min_grad;
+    k.min_tensor = p->min_tensor;
+    k.a11 = p->a11;
+    k.a004 = p->a004;
+    return k;
+}
+
+constexpr float kEps = 1e-7f;  // Python/vfs.cu:430,518
+
+inline dim3 grid2d(int H, int W)
+{
+    return dim3((W + VFS_BLOCK_X - 1) / VFS_BLOCK_X, (H + VFS_BLOCK_Y - 1) / VFS_BLOCK_Y, 1);
+}
+
+inline dim3 block2d() { return dim3(VFS_BLOCK_X, VFS_BLOCK_Y, 1); }
+
+}  // namespace vfs
+
+// Status helpers for the C ABI.
+#define VFS_CHECK_DIMS(H, W) \
+    do { if ((H) <= 0 || (W) <= 0 || (int64_t)(H) * (int64_t)(W) > (int64_t)1 << 30) return VFS_EINVAL; } while (0)
+#define VFS_CHECK_PTR(p) \
+    do { if ((p) == nullptr) return VFS_EINVAL; } while (0)
+#define VFS_LAUNCH_STATUS() \
+    do { hipError_t e_ = hipGetLastError(); return e_ == hipSuccess ? 0 : static_cast<int>(e_); } while (0)

@@ -1,0 +1,213 @@
+"""Seeded synthetic fisheye stereo pairs with known depth.
+
+Scene model (one pair):
+
+* equidistant camera r = f*theta with f = min(H, W) / pi (180 deg FOV), principal
+  point (W/2, H/2) (the convention of Python/vfs_icra_dataset.py:42-44);
+* range field D (metres along the ray) = base + Gaussian bumps + one planar
+  step, clipped; the z-depth that triangulation returns is D * cos(theta);
+* texture: multi-octave value noise, quantised to uint8;
+* second view: a point P seen by the reference camera is seen by the other
+  camera at R (P + t); the other image is rendered by inverse warping through
+  the true range field (fixed-point inversion of the forward map), plus
+  Gaussian noise, then quantised to uint8;
+* mask: the 180 deg image circle with an outer ring of ``margin`` pixels forced
+  to zero, so that every pyramid level keeps a zero border ring.
+
+This is input generation for tests and benchmarks; it is plain NumPy and runs
+on the host.
+"""
+
+import math
+
+import numpy as np
+
+
+def _project(xyz, f, cx, cy):
+    r = np.hypot(xyz[..., 0], xyz[..., 1])
+    th = np.arctan2(r, xyz[..., 2])
+    ph = np.arctan2(xyz[..., 1], xyz[..., 0])
+    return f * th * np.cos(ph) + cx, f * th * np.sin(ph) + cy
+
+
+def _unproject(x, y, f, cx, cy):
+    dx = x - cx
+    dy = y - cy
+    th = np.hypot(dx, dy) / f
+    ph = np.arctan2(dy, dx)
+    st = np.sin(th)
+    return np.stack([st * np.cos(ph), st * np.sin(ph), np.cos(th)], -1)
+
+
+def rotation_about_axis(axis, angle_rad):
+    """Rodrigues rotation matrix."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(angle_rad) * K + (1 - math.cos(angle_rad)) * (K @ K)
+
+
+class ValueNoise:
+    """Multi-octave value noise evaluated at continuous pixel coordinates."""
+
+    def __init__(self, rng, base_cell, octaves=5, lattice=256):
+        self.cells = [base_cell / (2 ** o) for o in range(octaves)]
+        self.amps = [0.5 ** o for o in range(octaves)]
+        self.tables = [rng.random((lattice, lattice)) for _ in range(octaves)]
+        self.lattice = lattice
+
+    def __call__(self, x, y):
+        acc = np.zeros(np.broadcast(x, y).shape)
+        n = self.lattice
+        for cell, amp, tab in zip(self.cells, self.amps, self.tables):
+            gx = x / cell
+            gy = y / cell
+            x0 = np.floor(gx)
+            y0 = np.floor(gy)
+            fx = gx - x0
+            fy = gy - y0
+            sx = fx * fx * (3 - 2 * fx)
+            sy = fy * fy * (3 - 2 * fy)
+            ix = np.mod(x0.astype(np.int64), n)
+            iy = np.mod(y0.astype(np.int64), n)
+            ix1 = (ix + 1) % n
+            iy1 = (iy + 1) % n
+            top = tab[iy, ix] * (1 - sx) + tab[iy, ix1] * sx
+            bot = tab[iy1, ix] * (1 - sx) + tab[iy1, ix1] * sx
+            acc += amp * (top * (1 - sy) + bot * sy)
+        return acc / sum(self.amps)
+
+
+class RangeField:
+    """D(x, y) = base + sum of Gaussian bumps + a planar step, clipped to [dmin, dmax]."""
+
+    def __init__(self, rng, H, W, base=4.0, n_bumps=8, amp=1.5, sigma_px=(10.0, 40.0),
+                 step=1.0, clip=(1.0, 10.0)):
+        s = min(H, W) / 256.0
+        self.base = base
+        self.cx = rng.uniform(0, W, n_bumps)
+        self.cy = rng.uniform(0, H, n_bumps)
+        self.amp = rng.uniform(-amp, amp, n_bumps)
+        self.sig = rng.uniform(sigma_px[0], sigma_px[1], n_bumps) * s
+        ang = rng.uniform(0, 2 * math.pi)
+        self.nx, self.ny = math.cos(ang), math.sin(ang)
+        self.off = rng.uniform(-0.25, 0.25) * min(H, W)
+        self.px, self.py = W / 2.0, H / 2.0
+        self.step = step * (1 if rng.random() < 0.5 else -1)
+        self.clip = clip
+
+    def __call__(self, x, y):
+        d = np.full(np.broadcast(x, y).shape, self.base)
+        for cx, cy, a, sg in zip(self.cx, self.cy, self.amp, self.sig):
+            d = d + a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * sg * sg))
+        side = (x - self.px) * self.nx + (y - self.py) * self.ny - self.off
+        d = d + np.where(side > 0, self.step, 0.0)
+        return np.clip(d, *self.clip)
+
+
+def circle_mask(H, W, margin, radius=None):
+    cx, cy = W / 2.0, H / 2.0
+    r = min(H, W) / 2.0 if radius is None else radius
+    Y, X = np.mgrid[0:H, 0:W]
+    m = ((X - cx) ** 2 + (Y - cy) ** 2) <= r * r
+    if margin > 0:
+        m[:margin, :] = False
+        m[-margin:, :] = False
+        m[:, :margin] = False
+        m[:, -margin:] = False
+    return (m.astype(np.uint8) * 255)
+
+
+def make_pair(H, W, seed, t=(0.2, 0.0, 0.0), rot_deg=1.0, rot_random=False, n_levels=3,
+              depth_base=4.0, depth_clip=(1.0, 10.0), contrast=1.0, noise_std=1.0 / 255.0,
+              fp_iters=30):
+    """Return a dict describing one seeded synthetic pair.
+
+    Keys: img_ref, img_other (H, W) uint8; mask (H, W) uint8 in {0, 255}; range_gt, z_gt
+    (H, W) float64; R (3, 3); t (3, 1); cam (f, cx, cy).
+    ``rot_random`` draws the rotation angle uniformly in [0, rot_deg].
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    f = min(H, W) / math.pi
+    cx, cy = W / 2.0, H / 2.0
+    axis = rng.normal(size=3)
+    ang = math.radians(rng.uniform(0, rot_deg) if rot_random else rot_deg)
+    R = rotation_about_axis(axis, ang)
+    tv = np.asarray(t, dtype=np.float64).reshape(3)
+    depth = RangeField(rng, H, W, base=depth_base, clip=depth_clip)
+    tex = ValueNoise(rng, base_cell=max(8.0, min(H, W) / 8.0))
+    margin = 2 ** (n_levels - 1) + 1
+
+    Y, X = np.mgrid[0:H, 0:W].astype(np.float64)
+    lo = 0.5 - 0.5 * contrast
+
+    def shade(x, y):
+        return lo + contrast * tex(x, y)
+
+    i0 = shade(X, Y)
+
+    def forward(x, y):
+        d = _unproject(x, y, f, cx, cy)
+        P = depth(x, y)[..., None] * d
+        Q = (P + tv) @ R.T
+        return _project(Q, f, cx, cy)
+
+    # inverse warp: for each pixel x' of the other view solve forward(x) = x'
+    xs, ys = X.copy(), Y.copy()
+    for _ in range(fp_iters):
+        fx, fy = forward(xs, ys)
+        xs = xs + (X - fx)
+        ys = ys + (Y - fy)
+    i1 = shade(xs, ys)
+
+    def quant(img):
+        img = img + rng.normal(0.0, noise_std, size=img.shape)
+        return np.clip(np.rint(img * 255.0), 0, 255).astype(np.uint8)
+
+    img_ref = quant(i0)
+    img_other = quant(i1)
+    rng_gt = depth(X, Y)
+    theta = np.hypot(X - cx, Y - cy) / f
+    return dict(img_ref=img_ref, img_other=img_other,
+                mask=circle_mask(H, W, margin), range_gt=rng_gt, z_gt=rng_gt * np.cos(theta),
+                R=R, t=tv.reshape(3, 1), cam=(f, cx, cy))
+
+
+# Named workloads (sizes and solver schedules) used by tests and the benchmark.
+WORKLOADS = {
+    "plain_small_256": dict(H=256, W=256, t=(0.2, 0.0, 0.0), rot_deg=1.0, rot_random=False,
+                            nLevel=3, nWarpIters=10, nSolverIters=20,
+                            Lambda=50.0, alpha0=5.0, alpha1=1.0, beta=10.0, gamma=0.5),
+    "plain_full_800": dict(H=800, W=800, t=(0.3, 0.0, 0.0), rot_deg=2.0, rot_random=True,
+                           nLevel=5, nWarpIters=20, nSolverIters=50,
+                           Lambda=50.0, alpha0=5.0, alpha1=1.0, beta=10.0, gamma=0.5),
+}
+
+
+def workload_pair(name, seed=0):
+    w = WORKLOADS[name]
+    return make_pair(w["H"], w["W"], seed, t=w["t"], rot_deg=w["rot_deg"], rot_random=w["rot_random"],
+                     n_levels=w["nLevel"])
+
+
+def sanitize_field(tvf):
+    """Zero the non-finite entries of a translation vector field.
+
+    With a 180 deg FOV and a translation in the image plane, the ray -t/|t| lies
+    on the image circle's rim: there (f(XYZ + t) - f(XYZ - t)) / 2 is the zero
+    vector and the unit normalisation of Python/vector_field.py:46-48 yields
+    0/0.  Bilinear sampling would spread that NaN into unmasked pixels through
+    the pyramid and the warp taps, so inputs built here carry 0 instead.
+    """
+    tvf = np.array(tvf, dtype=np.float32, copy=True)
+    tvf[~np.isfinite(tvf)] = 0.0
+    return tvf
+
+
+def apply_workload_params(param, name):
+    """Set the solver fields of a vfs_params-like object for a named workload (fScale 2)."""
+    w = WORKLOADS[name]
+    for k in ("nLevel", "nWarpIters", "nSolverIters", "Lambda", "alpha0", "alpha1", "beta", "gamma"):
+        setattr(param, k, w[k])
+    param.fScale = 2.0
+    return param
