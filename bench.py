@@ -160,6 +160,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--save-flow", default=None, help="write the last step's flow (H, W, 2) to this .npy file")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every pair is independent and this build runs on a single MI355X; "
@@ -194,6 +195,8 @@ def main():
     dt = time.perf_counter() - t0
     maps_s = a.steps / dt
 
+    if a.save_flow:
+        np.save(a.save_flow, v.warpUV.cpu().numpy())
     # accuracy of the timed result against the synthetic ground truth (not a parity check)
     pair = ctx["pair"]
     m = pair["mask"] > 0

@@ -32,11 +32,31 @@ Parity status of each function is pinned by tests/test_oracle.py (closed
 forms, invariants, brute force); see the test names cited per function.
 """
 
-import math
+import contextlib
 
 import numpy as np
 
 F64 = np.float64
+_DTYPE = [np.float64]
+
+
+@contextlib.contextmanager
+def precision(dtype):
+    """Evaluate the oracle in ``dtype`` inside the block (float64 outside).
+
+    float32 gives a restatement of the same arithmetic at single precision; tests
+    use it to measure how much a pure change of rounding moves the solve
+    (its conditioning), which sets the fair tolerance for an fp32 implementation.
+    """
+    _DTYPE.append(np.dtype(dtype).type)
+    try:
+        yield
+    finally:
+        _DTYPE.pop()
+
+
+def _ft():
+    return _DTYPE[-1]
 
 # Hidden constants of the reference (Python/vfs.py:52-54,62-63; Python/vfs.cu:430,518).
 ETA_P = 3.0
@@ -62,10 +82,10 @@ def sample_linear(img, x, y):
 
     Stands for ``tex2D`` on a Linear/Mirror/normalised texture (Python/vfs.py:96-98).
     """
-    img = np.asarray(img, dtype=F64)
+    img = np.asarray(img, dtype=_ft())
     H, W = img.shape[:2]
-    x = np.asarray(x, dtype=F64)
-    y = np.asarray(y, dtype=F64)
+    x = np.asarray(x, dtype=_ft())
+    y = np.asarray(y, dtype=_ft())
     xc = np.clip(np.nan_to_num(x, nan=0.0), -1e7, 1e7)
     yc = np.clip(np.nan_to_num(y, nan=0.0), -1e7, 1e7)
     x0 = np.floor(xc)
@@ -116,7 +136,7 @@ def compute_pyramid_sizes(height, width, n_level, f_scale):
 
 def cast_and_scale(img_u8):
     """castAndScale, Python/vfs.cu:145-156: float(u8) / 256."""
-    return np.asarray(img_u8, dtype=np.uint8).astype(F64) / 256.0
+    return np.asarray(img_u8, dtype=np.uint8).astype(_ft()) / 256.0
 
 
 def downsample_bilinear(src, out_shape):
@@ -162,7 +182,7 @@ def create_mask_pyramid(mask, sizes):
 
 def create_vector_field_pyramid(translation, sizes):
     """Python/vfs.py:189-218 (tvf pyramid only; the calibration field stays at full size)."""
-    tr = [np.asarray(translation, dtype=F64)]
+    tr = [np.asarray(translation, dtype=_ft())]
     for sz in sizes[1:]:
         tr.append(downsample_bilinear(tr[-1], sz))
     return tr
@@ -192,7 +212,7 @@ def create_image_pyramid(img_ref, img_other, calibration, sizes):
 
 def warping_image(src, warp):
     """warpingImage, Python/vfs.cu:61-66,186-202: S_lin(src, X + w.x, Y + w.y)."""
-    warp = np.asarray(warp, dtype=F64)
+    warp = np.asarray(warp, dtype=_ft())
     H, W = warp.shape[:2]
     Hs, Ws = src.shape[:2]
     X, Y = grid(H, W)
@@ -212,7 +232,7 @@ def find_warping_vector(warp, tvf):
 
 def gaussian_masked(img, mask):
     """gaussianMasked, Python/vfs.cu:81-102,204-226: binomial 3x3 / 16 with mirror edges; 0 where mask == 0."""
-    img = np.asarray(img, dtype=F64)
+    img = np.asarray(img, dtype=_ft())
     v0 = img
     v1 = (_nbr(img, -1, -1) + _nbr(img, -1, 1) + _nbr(img, 1, -1) + _nbr(img, 1, 1))
     v2 = (_nbr(img, -1, 0) + _nbr(img, 1, 0) + _nbr(img, 0, -1) + _nbr(img, 0, 1))
@@ -236,7 +256,7 @@ def compute_tensor_eta(img, mask, beta, gamma, alpha0, alpha1):
     g = (I(X+1,Y+1) - I(X,Y+1), I(X+1,Y+1) - I(X+1,Y)); n = g/|g| (or (1,0) if |g| < 1e-8);
     T = w n n^T + n_perp n_perp^T with w = max(exp(-beta |g|^gamma), 1e-8); (a, b, c) = (T00, T11, T01).
     """
-    img = np.asarray(img, dtype=F64)
+    img = np.asarray(img, dtype=_ft())
     i11 = _nbr(img, 1, 1)
     gx = i11 - _nbr(img, 0, 1)
     gy = i11 - _nbr(img, 1, 0)
@@ -261,9 +281,9 @@ def compute_derivatives(warped_other, ref, tvf):
 
     e = tvf(X,Y)/|tvf(X,Y)|; Iu = [R(x-2e) - 8R(x-e) + 8R(x+e) - R(x+2e) + same on Ow] / 24; Iz = Ow(x) - R(x).
     """
-    ref = np.asarray(ref, dtype=F64)
-    ow = np.asarray(warped_other, dtype=F64)
-    tvf = np.asarray(tvf, dtype=F64)
+    ref = np.asarray(ref, dtype=_ft())
+    ow = np.asarray(warped_other, dtype=_ft())
+    tvf = np.asarray(tvf, dtype=_ft())
     H, W = ref.shape
     X, Y = grid(H, W)
     with np.errstate(invalid="ignore", divide="ignore"):
@@ -306,8 +326,8 @@ def update_dual_variables(p, q, sigma, u_, v_, abc, mask, alpha0, alpha1):
     """updateDualVariables, Python/vfs.cu:326-419.  Returns new (p, tp, q); inputs are not modified."""
     m = np.asarray(mask) != 0
     mE, mS = _nbr(m, 1, 0), _nbr(m, 0, 1)
-    u_ = np.asarray(u_, dtype=F64)
-    v_ = np.asarray(v_, dtype=F64)
+    u_ = np.asarray(u_, dtype=_ft())
+    v_ = np.asarray(v_, dtype=_ft())
     uE, uW, uS, uN = _nbr(u_, 1, 0), _nbr(u_, -1, 0), _nbr(u_, 0, 1), _nbr(u_, 0, -1)
     vE, vW, vS, vN = _nbr(v_, 1, 0), _nbr(v_, -1, 0), _nbr(v_, 0, 1), _nbr(v_, 0, -1)
     vx, vy = v_[..., 0], v_[..., 1]
@@ -396,7 +416,7 @@ def update_primal_variables(tau, mu, p, q, u, u_, v_, abc, eta, mask, alpha0, al
 
 def median_filter(u, mask, ksize=MEDIAN_KERNEL_SIZE):
     """medianFilter, Python/vfs.cu:583-639: k x k median with clamp-to-edge indices; masked pixels copy."""
-    u = np.asarray(u, dtype=F64)
+    u = np.asarray(u, dtype=_ft())
     H, W = u.shape
     h = ksize // 2
     pad = np.pad(u, h, mode="edge")
@@ -427,9 +447,9 @@ def upsampling(src, out_shape, f_scale):
 
 def triangulate_rays(rays_ref, rays_other, t, mask):
     """triangulateRays, Python/vfs.cu:707-745: Z = (t.x - t.z x') / (x' - x); masked -> (0, 0, -1)."""
-    rr = np.asarray(rays_ref, dtype=F64)
-    ro = np.asarray(rays_other, dtype=F64)
-    t = np.asarray(t, dtype=F64).reshape(-1)
+    rr = np.asarray(rays_ref, dtype=_ft())
+    ro = np.asarray(rays_other, dtype=_ft())
+    t = np.asarray(t, dtype=_ft()).reshape(-1)
     with np.errstate(invalid="ignore", divide="ignore"):
         xr = rr[..., 0] / rr[..., 2]
         yr = rr[..., 1] / rr[..., 2]
@@ -446,7 +466,7 @@ def triangulate_rays(rays_ref, rays_other, t, mask):
 
 def project(xyz, f, u0, v0):
     """EquidistantCamera::project: theta = atan2(|xy|, z), phi = atan2(y, x), uv = f theta (cos, sin) + c."""
-    xyz = np.asarray(xyz, dtype=F64)
+    xyz = np.asarray(xyz, dtype=_ft())
     r = np.hypot(xyz[..., 0], xyz[..., 1])
     th = np.arctan2(r, xyz[..., 2])
     ph = np.arctan2(xyz[..., 1], xyz[..., 0])
@@ -455,7 +475,7 @@ def project(xyz, f, u0, v0):
 
 def unproject(uv, f, u0, v0):
     """EquidistantCamera::unproject: theta = |uv - c| / f, X = (sin th cos ph, sin th sin ph, cos th)."""
-    uv = np.asarray(uv, dtype=F64)
+    uv = np.asarray(uv, dtype=_ft())
     x = uv[..., 0] - u0
     y = uv[..., 1] - v0
     th = np.hypot(x, y) / f
@@ -467,14 +487,14 @@ def get_rays(shape, f, u0, v0):
     """getRays, Python/template_cam.cu:25-37: unproject integer pixel coordinates (no +0.5)."""
     H, W = shape
     X, Y = grid(H, W)
-    return unproject(np.stack([X, Y], -1).astype(F64), f, u0, v0)
+    return unproject(np.stack([X, Y], -1).astype(_ft()), f, u0, v0)
 
 
 def get_grid_zero(shape):
     """Python/vector_field.py:58-62."""
     H, W = shape
     X, Y = grid(H, W)
-    return np.stack([X, Y], -1).astype(F64)
+    return np.stack([X, Y], -1).astype(_ft())
 
 
 def get_vector_fields(grid_uv, cam_ref, cam_other, R, t, t_scale=1.0):
@@ -484,14 +504,14 @@ def get_vector_fields(grid_uv, cam_ref, cam_other, R, t, t_scale=1.0):
     """
     if isinstance(t_scale, (int, float)):
         t_scale = (t_scale, t_scale)
-    t = np.asarray(t, dtype=F64).reshape(3)
+    t = np.asarray(t, dtype=_ft()).reshape(3)
     xyz = unproject(grid_uv, *cam_ref)
     xyz = xyz / np.linalg.norm(xyz, axis=2)[..., None]
     fwd = project(xyz + t_scale[0] * t, *cam_ref)
     bwd = project(xyz - t_scale[1] * t, *cam_ref)
     tvf = (fwd - bwd) / 2.0
     tvf = tvf / np.linalg.norm(tvf, axis=2)[..., None]
-    rxyz = xyz @ np.asarray(R, dtype=F64).T
+    rxyz = xyz @ np.asarray(R, dtype=_ft()).T
     cvf = project(rxyz, *cam_other) - grid_uv
     return tvf, cvf
 
@@ -553,15 +573,15 @@ def solve_stereo(img_ref, img_other, mask, tvf, cvf, prm, trace=None):
     sizes = compute_pyramid_sizes(prm.height, prm.width, prm.nLevel, prm.fScale)
     masks = create_mask_pyramid(mask, sizes)
     trs = create_vector_field_pyramid(tvf, sizes)
-    refs, others = create_image_pyramid(img_ref, img_other, np.asarray(cvf, F64), sizes)
+    refs, others = create_image_pyramid(img_ref, img_other, np.asarray(cvf, _ft()), sizes)
     taus, sigmas, mus = pd_schedule(prm.nSolverIters, prm.timeStepLambda)
     warp = u = u_ = None
     for k in range(len(sizes), 0, -1):
         s = sizes[k - 1]
         if k == len(sizes):
-            warp = np.zeros((*s, 2))
-            u = np.zeros(s)
-            u_ = np.zeros(s)
+            warp = np.zeros((*s, 2), _ft())
+            u = np.zeros(s, _ft())
+            u_ = np.zeros(s, _ft())
         msk = masks[k - 1]
         ref_s = gaussian_masked(refs[k - 1], msk)
         abc, eta = compute_tensor_eta(ref_s, msk, prm.beta, prm.gamma, prm.alpha0, prm.alpha1)
@@ -569,9 +589,9 @@ def solve_stereo(img_ref, img_other, mask, tvf, cvf, prm, trace=None):
             tv = find_warping_vector(warp, trs[k - 1])
             ow = warping_image(others[k - 1], warp)
             deriv = compute_derivatives(ow, refs[k - 1], trs[k - 1])
-            p = np.zeros((*s, 2))
-            q = np.zeros((*s, 4))
-            v_ = np.zeros((*s, 2))
+            p = np.zeros((*s, 2), _ft())
+            q = np.zeros((*s, 4), _ft())
+            v_ = np.zeros((*s, 2), _ft())
             u0 = u.copy()
             for ks in range(prm.nSolverIters):
                 p, tp, q = update_dual_variables(p, q, sigmas[ks], u_, v_, abc, msk, prm.alpha0, prm.alpha1)

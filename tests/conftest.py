@@ -37,3 +37,22 @@ def small_case():
     prm = O.OracleParams(Hh, Ww)
     S.apply_workload_params(prm, "plain_small_256")
     return dict(pair=pair, tvf=tvf, cvf=cvf, prm=prm)
+
+
+@pytest.fixture(scope="session")
+def small_oracle(small_case):
+    """float64 and float32 oracle solves of the 256x256 workload, with per-level traces."""
+    from oracle import vfs_oracle as O
+    c = small_case
+    args = (c["pair"]["img_ref"], c["pair"]["img_other"], c["pair"]["mask"], c["tvf"], c["cvf"], c["prm"])
+    tr64, tr32 = {}, {}
+    f64 = O.solve_stereo(*args, trace=tr64)
+    with O.precision(np.float32):
+        f32 = O.solve_stereo(*args, trace=tr32)
+    return dict(flow=f64, trace=tr64, flow32=f32, trace32=tr32)
+
+
+def flow_diff_stats(flow, ref, mask):
+    d = np.linalg.norm(np.asarray(flow, np.float64) - np.asarray(ref, np.float64), axis=-1)[np.asarray(mask) > 0]
+    return dict(median=float(np.median(d)), p90=float(np.percentile(d, 90)), p99=float(np.percentile(d, 99)),
+                max=float(d.max()))

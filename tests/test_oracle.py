@@ -382,11 +382,30 @@ def test_schedule_closed_form():
     assert mus[0] == np.float32(1 / np.float32(np.sqrt(1.7)))
 
 
-def test_oracle_solve_recovers_known_flow(small_case):
+def test_fp32_restatement_conditioning(small_case, small_oracle):
+    """How far a pure change of rounding (float64 -> float32, same formulas) moves the solve.
+
+    This sets the fair end-to-end tolerance of an fp32 implementation: the median
+    stays below 1e-4 px and the coarsest level agrees tightly, while the warp /
+    median / threshold iteration amplifies rounding at a minority of pixels.
+    """
+    from conftest import flow_diff_stats
+    m = small_case["pair"]["mask"]
+    st = flow_diff_stats(small_oracle["flow32"], small_oracle["flow"], m)
+    print("fp32 restatement vs float64 oracle:", st)
+    assert st["median"] <= 1e-4
+    assert st["p99"] <= 0.5
+    sizes = O.compute_pyramid_sizes(256, 256, 3, 2.0)
+    mk = O.create_mask_pyramid(m, sizes)[2]
+    c = flow_diff_stats(small_oracle["trace32"][2]["warp"], small_oracle["trace"][2]["warp"], mk)
+    assert c["p99"] <= 1e-3
+
+
+def test_oracle_solve_recovers_known_flow(small_case, small_oracle):
     """End-to-end sanity on the seeded 256x256 pair: the flow approaches the true flow."""
     c = small_case
-    pair, prm = c["pair"], c["prm"]
-    flow = O.solve_stereo(pair["img_ref"], pair["img_other"], pair["mask"], c["tvf"], c["cvf"], prm)
+    pair = c["pair"]
+    flow = small_oracle["flow"]
     Hh, Ww = flow.shape[:2]
     cam = pair["cam"]
     g = O.get_grid_zero((Hh, Ww))

