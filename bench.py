@@ -116,6 +116,44 @@ def dual_kernel_roofline(sizes, n_warp, n_iter, prm):
                 achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
 
 
+def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
+    """Average duration and algorithmic bandwidth of the blocked primal-dual kernel over one solve's launches.
+
+    Bytes per launch (compulsory traffic of the formulation): every pixel of the
+    tile extended by 2k (west, north) and k (east, south), clipped to the image,
+    reads 73 B (u, u_bar, v_bar, p, q = 40 B; abc, eta, deriv = 32 B; mask 1 B;
+    the warp's first launch reads 41 B, its v_bar, p, q being zero), and every
+    tile pixel writes 40 B.
+    """
+    import torch
+    import autotune
+    total_ms, total_launches, total_bytes = 0.0, 0, 0
+    per_level = []
+    for (hh, ww), (tx, ty, k) in zip(sizes, tiles):
+        nblk = -(-n_iter // k)
+
+        def ext_px():
+            tot = 0
+            for y0 in range(0, hh, ty):
+                ey = min(hh, y0 + ty + k) - max(0, y0 - 2 * k)
+                for x0 in range(0, ww, tx):
+                    tot += ey * (min(ww, x0 + tx + k) - max(0, x0 - 2 * k))
+            return tot
+
+        ext = ext_px()
+        warp_bytes = (41 * ext + 40 * hh * ww) + (nblk - 1) * (73 * ext + 40 * hh * ww)
+        ms = autotune.time_variant(hh, ww, n_iter, (tx, ty, k), reps=n_warp)
+        per_level.append(dict(size=[hh, ww], tile=[tx, ty, k], launches=nblk * n_warp, avg_us=1e3 * ms / nblk,
+                              gbs=warp_bytes / (ms * 1e-3) / 1e9))
+        total_ms += ms * n_warp
+        total_launches += nblk * n_warp
+        total_bytes += warp_bytes * n_warp
+    avg_s = total_ms * 1e-3 / total_launches
+    bytes_per_launch = total_bytes / total_launches
+    return dict(avg_us=avg_s * 1e6, bytes_per_launch=bytes_per_launch,
+                achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
+
+
 def cpu_oracle_baseline(ctx, name):
     """float64 NumPy oracle (1 thread) on the full pyramid with 1 of nWarpIters warps per level."""
     from oracle import vfs_oracle as O
@@ -156,7 +194,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain_full_800")
-    ap.add_argument("--solver", default="graph", choices=["graph", "eager"])
+    ap.add_argument("--solver", default="blocked", choices=["graph", "eager", "blocked", "blocked_eager"])
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the plain per-kernel path beside it")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -183,20 +222,30 @@ def main():
         v.solve_stereo(ctx["img_ref"], ctx["img_other"])
         return v.triangulate_flow(ctx["cam"], ctx["pair"]["t"])
 
-    log(f"[bench] warmup {a.warmup}")
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    log(f"[bench] timing {a.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        xyz = step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    maps_s = a.steps / dt
+    def timed(label):
+        log(f"[bench] {label}: warmup {a.warmup}")
+        for _ in range(a.warmup):
+            step()
+        torch.cuda.synchronize()
+        log(f"[bench] {label}: timing {a.steps} steps")
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            out = step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
 
-    if a.save_flow:
-        np.save(a.save_flow, v.warpUV.cpu().numpy())
+    dt, xyz = timed(a.solver)
+    maps_s = a.steps / dt
+    flow_main = v.warpUV.cpu().numpy()
+    plain = None
+    if not a.no_compare and a.solver not in ("graph", "eager"):
+        v.solver = "graph"
+        dtp, _ = timed("plain per-kernel (graph)")
+        same = bool(np.array_equal(v.warpUV.cpu().numpy(), flow_main))
+        plain = dict(value=round(a.steps / dtp, 3), ms_per_step=round(1e3 * dtp / a.steps, 3),
+                     speedup_of_reported=round(dtp / dt, 2), flow_bitwise_equal=same)
+        v.solver = a.solver
+
     # accuracy of the timed result against the synthetic ground truth (not a parity check)
     pair = ctx["pair"]
     m = pair["mask"] > 0
@@ -207,9 +256,15 @@ def main():
 
     roof = None
     if not a.no_roofline:
-        log("[bench] dual-kernel roofline pass (HIP events)")
-        r = dual_kernel_roofline(sizes, w["nWarpIters"], w["nSolverIters"], v.param)
-        roof = dict(bound="hbm", kernel="k_update_dual", achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
+        if a.solver in ("blocked", "blocked_eager"):
+            log("[bench] blocked-kernel roofline pass (HIP events)")
+            r = blocked_kernel_roofline(sizes, v._tiles(), w["nWarpIters"], w["nSolverIters"])
+            kname = "k_pd_block"
+        else:
+            log("[bench] dual-kernel roofline pass (HIP events)")
+            r = dual_kernel_roofline(sizes, w["nWarpIters"], w["nSolverIters"], v.param)
+            kname = "k_update_dual"
+        roof = dict(bound="hbm", kernel=kname, achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
                     avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
                     per_level=r["per_level"])
@@ -234,7 +289,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": a.workload, "size": [w["H"], w["W"]], "levels": w["nLevel"],
                    "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
-                   "solver": f"plain per-kernel launches ({a.solver})", "pairs": 1},
+                   "solver": a.solver, "pairs": 1,
+                   "pd_tiles": [list(t) for t in v._tiles()] if a.solver.startswith("blocked") else None},
+        "plain_per_kernel": plain,
         "mpx_iter_per_s": round(px_iter * maps_s / 1e6, 1),
         "solver_effective_gbs": round(bytes_map * maps_s / 1e9, 1),
         "median_rel_depth_err_vs_gt": round(depth_err, 4),

@@ -225,6 +225,48 @@ int vfs_capture_begin(void** capture_stream);
 int vfs_capture_end(void* capture_stream, void** plan);
 int vfs_plan_destroy(void* plan);
 
+/* ---- temporally blocked primal-dual loop (no reference counterpart; the reference
+ * launches the three kernels of Python/vfs.py:505-511 once per iteration) ------- */
+
+/* Second state set for the ping-pong of the blocked loop, shapes as in vfs_level:
+ * u_th0 / u_th1 (H,W) thresholded u, u_bar2 (H,W), v_bar2 (H,W,2), p2 (H,W,2),
+ * q2 (H,W,4). */
+typedef struct vfs_pd_ws {
+    float* u_th0;
+    float* u_th1;
+    float* u_bar2;
+    float* v_bar2;
+    float* p2;
+    float* q2;
+} vfs_pd_ws;
+
+/* Tile variants compiled into the library: writes (tile_x, tile_y, k, threads)
+ * quadruples to out (4 ints each) and returns their number; out == NULL returns
+ * the count only. */
+int vfs_pd_variants(int* out, int max_variants);
+
+/* n_iter (1 <= n_iter <= k) primal-dual iterations of Python/vfs.py:498-515 in one
+ * launch of the (tile_x, tile_y, k) variant: reads the state (u_in, ub_in,
+ * vb_in, p_in, q_in; vb/p/q taken as zero when first != 0) and writes it to the
+ * out buffers, which must not alias the in buffers.  taus/sigmas/mus: n_iter
+ * host values. */
+int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* abc,
+                  const float* eta, const float* deriv, const float* u_in, const float* ub_in,
+                  const float* vb_in, const float* p_in, const float* q_in, float* u_out,
+                  float* ub_out, float* vb_out, float* p_out, float* q_out, int first,
+                  const float* taus, const float* sigmas, const float* mus, int n_iter,
+                  int tile_x, int tile_y, int k, const vfs_params* prm);
+
+/* One level of Python/vfs.py:473-521 with the blocked loop: Gaussian + tensor,
+ * then per warp: derivatives, ceil(n_iter / k) blocked launches, and one kernel
+ * for median + clamp + the next warp's warping vector and warped image.  Same
+ * results as vfs_solve_level_plain; lv->p, q, v_bar and ws are scratch, lv->tp,
+ * lv->u0 and lv->uf are unused. */
+int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* ws,
+                            const vfs_params* prm, const float* taus, const float* sigmas,
+                            const float* mus, int n_iter, int n_warp, int tile_x, int tile_y,
+                            int k);
+
 /* Library version string (static storage). */
 const char* vfs_version(void);
 

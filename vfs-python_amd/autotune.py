@@ -1,0 +1,159 @@
+"""Autotuner of the temporally blocked primal-dual kernel (tile shape x k).
+
+For each pyramid-level size the tuner times every compiled variant
+(``vfs_hip.pd_variants()``: tile {16, 32, 64} x {16, 32} x k {1, 2, 4, 8}) on one
+warp's worth of iterations (ceil(n_iter / k) launches, ping-ponged state, HIP
+events on the launch stream) and keeps the fastest.  Results are stored in a
+JSON table next to this file (``autotune_gfx950.json``) and read back by
+``tile_for``; sizes missing from the table use the entry with the nearest pixel
+count, or a fixed default before any tuning.
+
+Usage on the GPU:  python autotune.py --sizes 800x800,400x400,... --iters 50
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "autotune_gfx950.json")
+
+_table = None
+
+
+def _load():
+    global _table
+    if _table is None:
+        _table = {}
+        if os.path.exists(TABLE):
+            with open(TABLE) as f:
+                for row in json.load(f).get("entries", []):
+                    _table[(int(row["h"]), int(row["w"]), int(row["iters"]))] = tuple(row["best"][:3])
+    return _table
+
+
+def default_tile(h, w):
+    """Choice before any tuning: larger k where launches dominate (small levels)."""
+    n = h * w
+    if n >= 250_000:
+        return (32, 32, 4)
+    if n >= 40_000:
+        return (32, 16, 8)
+    return (16, 16, 8)
+
+
+def tile_for(h, w, n_iter):
+    """(tile_x, tile_y, k) for a level of h x w pixels and n_iter iterations per warp."""
+    t = _load()
+    if (h, w, n_iter) in t:
+        return t[(h, w, n_iter)]
+    if t:
+        best = min(t.items(), key=lambda kv: (abs(kv[0][0] * kv[0][1] - h * w), abs(kv[0][2] - n_iter)))
+        return best[1]
+    return default_tile(h, w)
+
+
+def time_variant(h, w, n_iter, variant, reps=5, seed=0):
+    """Mean time (ms) of one warp's n_iter blocked iterations at h x w with ``variant``."""
+    import numpy as np
+    import torch
+    import vfs_hip as H
+
+    class P:
+        beta, gamma, alpha0, alpha1, Lambda, limitRange, fScale = 10.0, 0.5, 5.0, 1.0, 50.0, 0.1, 2.0
+
+    prm = H.make_params(P)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    dev = dict(device="cuda")
+    mask = (torch.rand((h, w), generator=g, **dev) > 0.1).to(torch.uint8)
+    abc = torch.rand((h, w, 3), generator=g, **dev)
+    eta = torch.rand((h, w, 3), generator=g, **dev) * 20 + 1
+    deriv = torch.randn((h, w, 2), generator=g, **dev) * 0.1
+    sets = []
+    for _ in range(2):
+        sets.append(dict(u=torch.zeros((h, w), **dev), ub=torch.zeros((h, w), **dev), vb=torch.zeros((h, w, 2), **dev),
+                         p=torch.zeros((h, w, 2), **dev), q=torch.zeros((h, w, 4), **dev)))
+    u0 = torch.zeros((h, w), **dev)
+    taus, sigmas, mus = H.pd_schedule(n_iter, 1.0)
+    tx, ty, k = variant[:3]
+    s = H.stream()
+    fn = H.lib().vfs_pd_blocks
+
+    def run_warp():
+        cur, done, first = 0, 0, True
+        while done < n_iter:
+            kc = min(k, n_iter - done)
+            a, b = sets[cur], sets[1 - cur]
+            sl = slice(done, done + kc)
+            rc = fn(s, h, w, H.ptr(mask), H.ptr(abc), H.ptr(eta), H.ptr(deriv), H.ptr(u0 if first else a["u"]),
+                    H.ptr(a["ub"]), H.ptr(a["vb"]), H.ptr(a["p"]), H.ptr(a["q"]), H.ptr(b["u"]), H.ptr(b["ub"]),
+                    H.ptr(b["vb"]), H.ptr(b["p"]), H.ptr(b["q"]), 1 if first else 0,
+                    np.ascontiguousarray(taus[sl]).ctypes.data_as(ctypes.c_void_p),
+                    np.ascontiguousarray(sigmas[sl]).ctypes.data_as(ctypes.c_void_p),
+                    np.ascontiguousarray(mus[sl]).ctypes.data_as(ctypes.c_void_p), kc, tx, ty, k, ctypes.byref(prm))
+            H.check(rc, "vfs_pd_blocks")
+            cur, done, first = 1 - cur, done + kc, False
+
+    run_warp()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run_warp()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(sizes, n_iter, reps=5, log=print):
+    import vfs_hip as H
+    variants = H.pd_variants()
+    entries = []
+    for (h, w) in sizes:
+        results = []
+        for v in variants:
+            ms = time_variant(h, w, n_iter, v, reps=reps)
+            results.append((ms, v))
+            log(f"[autotune] {h}x{w} iters {n_iter} tile {v[0]}x{v[1]} k {v[2]} threads {v[3]}: {ms * 1e3:.1f} us/warp")
+        results.sort()
+        best = results[0][1]
+        entries.append(dict(h=h, w=w, iters=n_iter, best=list(best), us_per_warp=round(results[0][0] * 1e3, 2),
+                            all=[dict(tile=list(v), us=round(ms * 1e3, 2)) for ms, v in results]))
+        log(f"[autotune] {h}x{w}: best tile {best[0]}x{best[1]} k {best[2]} ({results[0][0] * 1e3:.1f} us/warp)")
+    return entries
+
+
+def save(entries, path=TABLE):
+    old = []
+    if os.path.exists(path):
+        with open(path) as f:
+            old = json.load(f).get("entries", [])
+    keep = {(e["h"], e["w"], e["iters"]): e for e in old}
+    for e in entries:
+        keep[(e["h"], e["w"], e["iters"])] = e
+    with open(path, "w") as f:
+        json.dump({"arch": "gfx950", "entries": sorted(keep.values(), key=lambda e: -e["h"] * e["w"])}, f, indent=1)
+    global _table
+    _table = None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sizes", default="800x800,400x400,200x200,100x100,50x50")
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--out", default=TABLE)
+    a = ap.parse_args()
+    sizes = [tuple(int(x) for x in s.split("x")) for s in a.sizes.split(",")]
+    entries = tune(sizes, a.iters, a.reps, log=lambda *m: print(*m, file=sys.stderr, flush=True))
+    save(entries, a.out)
+    print(json.dumps([dict(h=e["h"], w=e["w"], best=e["best"], us_per_warp=e["us_per_warp"]) for e in entries]))
+
+
+if __name__ == "__main__":
+    if HERE not in sys.path:
+        sys.path.insert(0, HERE)
+    main()

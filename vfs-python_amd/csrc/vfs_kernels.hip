@@ -4,6 +4,7 @@
 // row access).  Each kernel cites the reference kernel whose semantics it
 // implements.
 #include "vfs_device.h"
+#include "vfs_pd.h"
 
 using namespace vfs;
 
@@ -214,47 +215,15 @@ __global__ void __launch_bounds__(256) k_update_dual(float2* __restrict__ p, flo
     }
     const int iE = Y * W + clamp_idx(X + 1, W), iW = Y * W + clamp_idx(X - 1, W);
     const int iS = clamp_idx(Y + 1, H) * W + X, iN = clamp_idx(Y - 1, H) * W + X;
-    const float uc = u[idx];
-    const float2 vc = v[idx];
-    float2 gU;
-    float4 gV;
-    if (mask[iE] != 0) {  // E
-        const float2 ve = v[iE];
-        gU.x = u[iE] - uc - vc.x;
-        gV.x = ve.x - vc.x;
-        gV.w = ve.y - vc.y;
-    } else {  // W
-        const float2 vw = v[iW];
-        gU.x = uc - u[iW] - vc.x;
-        gV.x = vc.x - vw.x;
-        gV.w = vc.y - vw.y;
-    }
-    if (mask[iS] != 0) {  // S
-        const float2 vs = v[iS];
-        gU.y = u[iS] - uc - vc.y;
-        gV.y = vs.y - vc.y;
-        gV.z = vs.x - vc.x;
-    } else {  // N
-        const float2 vn = v[iN];
-        gU.y = uc - u[iN] - vc.y;
-        gV.y = vc.y - vn.y;
-        gV.z = vc.x - vn.x;
-    }
-    const float a = abc[3 * idx + 0], b = abc[3 * idx + 1], c = abc[3 * idx + 2];
-    const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
-    float scale = alpha1 * sigma / eta_p;
-    const float2 pc = p[idx];
-    const float2 up = make_float2(fmaf(scale, tu.x, pc.x), fmaf(scale, tu.y, pc.y));
-    float nrm = fmaxf(hypotf(up.x, up.y), 1.0f);
-    const float2 pn = make_float2(up.x / nrm, up.y / nrm);
+    const bool fx = mask[iE] != 0, fy = mask[iS] != 0;  // E / S present: forward differences
+    const int ix = fx ? iE : iW, iy = fy ? iS : iN;
+    float2 pn, tpn;
+    float4 qn;
+    dual_pixel(u[idx], v[idx], u[ix], v[ix], fx, u[iy], v[iy], fy, abc[3 * idx + 0], abc[3 * idx + 1],
+               abc[3 * idx + 2], p[idx], q[idx], sigma, alpha0, alpha1, eta_p, eta_q, pn, tpn, qn);
     p[idx] = pn;
-    tp[idx] = make_float2(fmaf(a, pn.x, c * pn.y), fmaf(c, pn.x, b * pn.y));
-    scale = alpha0 * sigma / eta_q;
-    const float4 qc = q[idx];
-    const float4 uq = make_float4(fmaf(scale, gV.x, qc.x), fmaf(scale, gV.y, qc.y),
-                                  fmaf(scale, gV.z, qc.z), fmaf(scale, gV.w, qc.w));
-    nrm = fmaxf(norm4df(uq.x, uq.y, uq.z, uq.w), 1.0f);
-    q[idx] = make_float4(uq.x / nrm, uq.y / nrm, uq.z / nrm, uq.w / nrm);
+    tp[idx] = tpn;
+    q[idx] = qn;
 }
 
 // l1Thresholding, Python/vfs.cu:421-505.
@@ -271,36 +240,9 @@ __global__ void __launch_bounds__(256) k_l1_thresholding(float* out, float tau, 
     const int iW = Y * W + clamp_idx(X - 1, W), iN = clamp_idx(Y - 1, H) * W + X;
     const bool mE = mask[Y * W + clamp_idx(X + 1, W)] != 0, mW = mask[iW] != 0;
     const bool mS = mask[clamp_idx(Y + 1, H) * W + X] != 0, mN = mask[iN] != 0;
-    const float2 tc = tp[idx];
-    float gx, gy;
-    if (mE && mW)
-        gx = tc.x - tp[iW].x;
-    else if (!mE)
-        gx = -tp[iW].x;
-    else
-        gx = tc.x;
-    if (mS && mN)
-        gy = tc.y - tp[iN].y;
-    else if (!mS)
-        gy = -tp[iN].y;
-    else
-        gy = tc.y;
     const float2 d = deriv[idx];
-    const float dIu = d.x, dIz = d.y;
-    const float e0 = eta[3 * idx];
-    const float tauEtaU = (fabsf(e0) < kEps) ? tau : tau / e0;
-    const float uc = u[idx];
-    const float uTauGtp = u_bar[idx] + tauEtaU * (gx + gy);
-    const float rho = dIu * (uTauGtp - uc) + dIz;
-    const float bound = lambda * tauEtaU * dIu * dIu;
-    float du = 0.0f;
-    if ((rho <= bound) && (rho >= -bound))
-        du = (fabsf(dIu) < kEps) ? (uTauGtp - uc) : ((uTauGtp - uc) - rho / dIu);
-    else if (rho < -bound)
-        du = (uTauGtp - uc) + lambda * tauEtaU * dIu;
-    else if (rho > bound)
-        du = (uTauGtp - uc) - lambda * tauEtaU * dIu;
-    out[idx] = uc + du;
+    out[idx] = l1_pixel(tp[idx], tp[iW].x, tp[iN].y, mE, mW, mS, mN, d.x, d.y, eta[3 * idx], u[idx], u_bar[idx], tau,
+                        lambda, kEps);
 }
 
 // updatePrimalVariables, Python/vfs.cu:507-581.
@@ -317,39 +259,18 @@ __global__ void __launch_bounds__(256) k_update_primal(float* u_out, float2* v_o
         return;
     }
     const int iW = Y * W + clamp_idx(X - 1, W), iN = clamp_idx(Y - 1, H) * W + X;
-    const float4 qc = q[idx];
-    float4 gQ = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (mask[Y * W + clamp_idx(X + 1, W)] != 0) {  // E
-        gQ.x = qc.x;
-        gQ.w = qc.w;
-    }
-    if (mask[iW] != 0) {  // W
-        const float4 qw = q[iW];
-        gQ.x -= qw.x;
-        gQ.w -= qw.w;
-    }
-    if (mask[clamp_idx(Y + 1, H) * W + X] != 0) {  // S
-        gQ.y = qc.y;
-        gQ.z = qc.z;
-    }
-    if (mask[iN] != 0) {  // N
-        const float4 qn = q[iN];
-        gQ.y -= qn.y;
-        gQ.z -= qn.z;
-    }
-    const float a = abc[3 * idx + 0], b = abc[3 * idx + 1], c = abc[3 * idx + 2];
-    const float2 pc = p[idx];
-    const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
-    const float e1 = eta[3 * idx + 1], e2 = eta[3 * idx + 2];
-    const float tauEta1 = (fabsf(e1) < kEps) ? tau : (tau / e1);
-    const float tauEta2 = (fabsf(e2) < kEps) ? tau : (tau / e2);
-    const float2 vb = v_bar[idx];
-    const float vx = fmaf(tauEta1, fmaf(alpha1, tu.x, alpha0 * (gQ.x + gQ.z)), vb.x);
-    const float vy = fmaf(tauEta2, fmaf(alpha1, tu.y, alpha0 * (gQ.w + gQ.y)), vb.y);
-    const float uc = u[idx];
-    const float ub = u_bar[idx];
-    u_out[idx] = fmaf(mu, uc - ub, uc);
-    v_out[idx] = make_float2(fmaf(mu, vx - vb.x, vx), fmaf(mu, vy - vb.y, vy));
+    const bool mE = mask[Y * W + clamp_idx(X + 1, W)] != 0, mW = mask[iW] != 0;
+    const bool mS = mask[clamp_idx(Y + 1, H) * W + X] != 0, mN = mask[iN] != 0;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 qW = mW ? q[iW] : zero;
+    const float4 qN = mN ? q[iN] : zero;
+    float ub;
+    float2 vb;
+    primal_pixel(q[idx], qW, qN, mE, mW, mS, mN, abc[3 * idx + 0], abc[3 * idx + 1], abc[3 * idx + 2], p[idx],
+                 eta[3 * idx + 1], eta[3 * idx + 2], v_bar[idx], u[idx], u_bar[idx], tau, mu, alpha0, alpha1, kEps, ub,
+                 vb);
+    u_out[idx] = ub;
+    v_out[idx] = vb;
 }
 
 // medianFilter, Python/vfs.cu:583-639: exact (K*K/2)-th order statistic of the

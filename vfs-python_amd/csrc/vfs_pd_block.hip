@@ -1,0 +1,559 @@
+// Temporally blocked primal-dual iterations (K iterations per launch) and the
+// fused per-warp kernels of the blocked level loop, gfx950.
+//
+// One iteration (dual -> threshold -> primal, Python/vfs.py:498-515) reads, at a
+// pixel, u_bar / v_bar at its four neighbours (dual), tp at its west and north
+// neighbours (threshold) and q at its west and north neighbours (primal).  The
+// region on which the state is exact therefore shrinks by one pixel on the
+// east/south side and by two on the west/north side per iteration.  A workgroup
+// that must produce a TX x TY tile after K iterations loads the tile extended by
+// 2K (west, north) and K (east, south), keeps each owned pixel's state and
+// constants in registers, exchanges the neighbour fields through LDS (two
+// barriers per iteration), and writes back only the tile.  Neighbouring tiles
+// read their halos from the same input buffers, so the state is ping-ponged
+// between two buffer sets across launches.
+//
+// The per-pixel arithmetic is the same inline code as the per-kernel path
+// (vfs_pd.h).
+#include <new>
+
+#include "vfs_device.h"
+#include "vfs_pd.h"
+
+using namespace vfs;
+
+namespace {
+
+constexpr int KMAX = 8;
+
+struct PDSched {
+    float tau[KMAX];
+    float sigma[KMAX];
+    float mu[KMAX];
+};
+
+struct PDConsts {
+    float alpha0, alpha1, eta_p, eta_q, lambda;
+};
+
+// One pyramid level of one pair for one launch: constant inputs, the state read
+// (in_*) and the state written (out_*), all C-contiguous (H, W[, C]) float32.
+struct PDLevel {
+    int H, W;
+    int tiles_x, tiles_y;
+    int first;  // 1: p = q = v_bar = 0 on input (the reset at warp start)
+    const uint8_t* mask;
+    const float* abc;
+    const float* eta;
+    const float* deriv;
+    const float* u_in;
+    const float* ub_in;
+    const float* vb_in;
+    const float* p_in;
+    const float* q_in;
+    float* u_out;
+    float* ub_out;
+    float* vb_out;
+    float* p_out;
+    float* q_out;
+};
+
+enum : unsigned {
+    F_ACT = 1u,     // inside the image and inside the mask
+    F_ME = 2u,      // mask at the east neighbour (mirror at the border)
+    F_MW = 4u,
+    F_MS = 8u,
+    F_MN = 16u,
+    F_HE = 32u,     // east neighbour is a different pixel of the loaded region
+    F_HW = 64u,
+    F_HS = 128u,
+    F_HN = 256u,
+    F_IMG = 512u,   // inside the image
+};
+
+template <int TX, int TY, int K, int NT>
+__device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, int k_count, const PDSched& sc,
+                                              const PDConsts& cst, float* __restrict__ sA, float* __restrict__ sB)
+{
+    constexpr int EX = TX + 3 * K, EY = TY + 3 * K, E = EX * EY;
+    constexpr int S = (E + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const int H = L.H, W = L.W;
+    const int ox = tx * TX - 2 * K, oy = ty * TY - 2 * K;
+
+    float u[S], ub[S], vbx[S], vby[S], px[S], py[S], qx[S], qy[S], qz[S], qw[S];
+    float ca[S], cb[S], cc[S], e0[S], e1[S], e2[S], iu[S], iz[S];
+    unsigned fl[S];
+
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int e = tid + NT * j;
+        const int ex = e % EX, ey = e / EX;
+        const int gx = ox + ex, gy = oy + ey;
+        unsigned f = 0;
+        u[j] = ub[j] = vbx[j] = vby[j] = px[j] = py[j] = qx[j] = qy[j] = qz[j] = qw[j] = 0.0f;
+        ca[j] = cb[j] = cc[j] = e0[j] = e1[j] = e2[j] = iu[j] = iz[j] = 0.0f;
+        if (e < E && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int idx = gy * W + gx;
+            f |= F_IMG;
+            if (gx < W - 1 && ex < EX - 1) f |= F_HE;
+            if (gx > 0 && ex > 0) f |= F_HW;
+            if (gy < H - 1 && ey < EY - 1) f |= F_HS;
+            if (gy > 0 && ey > 0) f |= F_HN;
+            // u_bar / v_bar are loaded at masked pixels too: the dual step of an
+            // inside pixel whose east (south) neighbour is masked reads the west
+            // (north) neighbour whether or not that one is masked.
+            ub[j] = L.ub_in[idx];
+            if (!L.first) {
+                const float2 v = reinterpret_cast<const float2*>(L.vb_in)[idx];
+                vbx[j] = v.x;
+                vby[j] = v.y;
+            }
+            if (L.mask[idx] != 0) {
+                f |= F_ACT;
+                if (L.mask[gy * W + clamp_idx(gx + 1, W)] != 0) f |= F_ME;
+                if (L.mask[gy * W + clamp_idx(gx - 1, W)] != 0) f |= F_MW;
+                if (L.mask[clamp_idx(gy + 1, H) * W + gx] != 0) f |= F_MS;
+                if (L.mask[clamp_idx(gy - 1, H) * W + gx] != 0) f |= F_MN;
+                u[j] = L.u_in[idx];
+                if (!L.first) {
+                    const float2 p = reinterpret_cast<const float2*>(L.p_in)[idx];
+                    const float4 q = reinterpret_cast<const float4*>(L.q_in)[idx];
+                    px[j] = p.x;
+                    py[j] = p.y;
+                    qx[j] = q.x;
+                    qy[j] = q.y;
+                    qz[j] = q.z;
+                    qw[j] = q.w;
+                }
+                ca[j] = L.abc[3 * idx + 0];
+                cb[j] = L.abc[3 * idx + 1];
+                cc[j] = L.abc[3 * idx + 2];
+                e0[j] = L.eta[3 * idx + 0];
+                e1[j] = L.eta[3 * idx + 1];
+                e2[j] = L.eta[3 * idx + 2];
+                const float2 d = reinterpret_cast<const float2*>(L.deriv)[idx];
+                iu[j] = d.x;
+                iz[j] = d.y;
+            }
+        }
+        fl[j] = f;
+        if (e < E) {
+            sA[e] = ub[j];
+            sA[E + e] = vbx[j];
+            sA[2 * E + e] = vby[j];
+        }
+    }
+
+    for (int k = 0; k < k_count; ++k) {
+        const float sigma = sc.sigma[k], tau = sc.tau[k], mu = sc.mu[k];
+        __syncthreads();  // sA (u_bar, v_bar) complete
+        // dual step
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int e = tid + NT * j;
+            if (e >= E) continue;
+            const unsigned f = fl[j];
+            float tpx = 0.0f, tpy = 0.0f;
+            if (f & F_ACT) {
+                const bool fx = (f & F_ME) != 0, fy = (f & F_MS) != 0;
+                const int eE = (f & F_HE) ? e + 1 : e, eW = (f & F_HW) ? e - 1 : e;
+                const int eS = (f & F_HS) ? e + EX : e, eN = (f & F_HN) ? e - EX : e;
+                const int ix = fx ? eE : eW, iy = fy ? eS : eN;
+                float2 pn, tpn;
+                float4 qn;
+                dual_pixel(ub[j], make_float2(vbx[j], vby[j]), sA[ix], make_float2(sA[E + ix], sA[2 * E + ix]), fx,
+                           sA[iy], make_float2(sA[E + iy], sA[2 * E + iy]), fy, ca[j], cb[j], cc[j],
+                           make_float2(px[j], py[j]), make_float4(qx[j], qy[j], qz[j], qw[j]), sigma, cst.alpha0,
+                           cst.alpha1, cst.eta_p, cst.eta_q, pn, tpn, qn);
+                px[j] = pn.x;
+                py[j] = pn.y;
+                tpx = tpn.x;
+                tpy = tpn.y;
+                qx[j] = qn.x;
+                qy[j] = qn.y;
+                qz[j] = qn.z;
+                qw[j] = qn.w;
+            } else {
+                px[j] = py[j] = qx[j] = qy[j] = qz[j] = qw[j] = 0.0f;
+            }
+            sB[e] = tpx;
+            sB[E + e] = tpy;
+            sB[2 * E + e] = qx[j];
+            sB[3 * E + e] = qy[j];
+            sB[4 * E + e] = qz[j];
+            sB[5 * E + e] = qw[j];
+        }
+        __syncthreads();  // sB (tp, q) complete
+        // threshold, then primal step (reads sB only; writes sA, last read before the barrier above)
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int e = tid + NT * j;
+            if (e >= E) continue;
+            const unsigned f = fl[j];
+            if (f & F_ACT) {
+                const int eW = (f & F_HW) ? e - 1 : e, eN = (f & F_HN) ? e - EX : e;
+                const bool mE = (f & F_ME) != 0, mW = (f & F_MW) != 0, mS = (f & F_MS) != 0, mN = (f & F_MN) != 0;
+                u[j] = l1_pixel(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j], iz[j],
+                                e0[j], u[j], ub[j], tau, cst.lambda, kEps);
+                const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const float4 qW = mW ? make_float4(sB[2 * E + eW], sB[3 * E + eW], sB[4 * E + eW], sB[5 * E + eW]) : zero;
+                const float4 qN = mN ? make_float4(sB[2 * E + eN], sB[3 * E + eN], sB[4 * E + eN], sB[5 * E + eN]) : zero;
+                float ubn;
+                float2 vbn;
+                primal_pixel(make_float4(qx[j], qy[j], qz[j], qw[j]), qW, qN, mE, mW, mS, mN, ca[j], cb[j], cc[j],
+                             make_float2(px[j], py[j]), e1[j], e2[j], make_float2(vbx[j], vby[j]), u[j], ub[j], tau,
+                             mu, cst.alpha0, cst.alpha1, kEps, ubn, vbn);
+                ub[j] = ubn;
+                vbx[j] = vbn.x;
+                vby[j] = vbn.y;
+            } else {
+                u[j] = ub[j] = vbx[j] = vby[j] = 0.0f;
+            }
+            sA[e] = ub[j];
+            sA[E + e] = vbx[j];
+            sA[2 * E + e] = vby[j];
+        }
+    }
+
+    // write back the tile interior
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int e = tid + NT * j;
+        if (e >= E) continue;
+        const int ex = e % EX, ey = e / EX;
+        if (ex < 2 * K || ex >= 2 * K + TX || ey < 2 * K || ey >= 2 * K + TY) continue;
+        if (!(fl[j] & F_IMG)) continue;
+        const int idx = (oy + ey) * W + (ox + ex);
+        L.u_out[idx] = u[j];
+        L.ub_out[idx] = ub[j];
+        reinterpret_cast<float2*>(L.vb_out)[idx] = make_float2(vbx[j], vby[j]);
+        reinterpret_cast<float2*>(L.p_out)[idx] = make_float2(px[j], py[j]);
+        reinterpret_cast<float4*>(L.q_out)[idx] = make_float4(qx[j], qy[j], qz[j], qw[j]);
+    }
+}
+
+template <int TX, int TY, int K, int NT>
+constexpr int lds_floats()
+{
+    return 9 * (TX + 3 * K) * (TY + 3 * K);
+}
+
+// Tiles of one level; blockIdx.x enumerates tiles row by row.
+template <int TX, int TY, int K, int NT>
+__global__ void __launch_bounds__(NT) k_pd_block(PDLevel L, int k_count, PDSched sc, PDConsts cst)
+{
+    constexpr int E = (TX + 3 * K) * (TY + 3 * K);
+    __shared__ float sA[3 * E];
+    __shared__ float sB[6 * E];
+    const int t = blockIdx.x;
+    pd_block_tile<TX, TY, K, NT>(L, t % L.tiles_x, t / L.tiles_x, k_count, sc, cst, sA, sB);
+}
+
+// Tiles of several levels / pairs in one launch.  tile_start[i] is the first
+// tile index of levels[i] (prefix sums, tile_start[n] = total).
+template <int TX, int TY, int K, int NT>
+__global__ void __launch_bounds__(NT) k_pd_block_batched(const PDLevel* __restrict__ levels,
+                                                         const int* __restrict__ tile_start, int n, int k_count,
+                                                         PDSched sc, PDConsts cst)
+{
+    constexpr int E = (TX + 3 * K) * (TY + 3 * K);
+    __shared__ float sA[3 * E];
+    __shared__ float sB[6 * E];
+    const int t = blockIdx.x;
+    int lo = 0, hi = n - 1;  // last i with tile_start[i] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const PDLevel& L = levels[lo];
+    const int lt = t - tile_start[lo];
+    pd_block_tile<TX, TY, K, NT>(L, lt % L.tiles_x, lt / L.tiles_x, k_count, sc, cst, sA, sB);
+}
+
+// ---------------------------------------------------------------------------
+// per-warp kernels of the blocked loop
+// ---------------------------------------------------------------------------
+
+// findWarpingVector + warpingImage at one pixel (Python/vfs.py:480,483).
+__device__ __forceinline__ void warp_prep_pixel(int X, int Y, int idx, float2 f, const float2* __restrict__ tvf,
+                                                const float* __restrict__ other, int H, int W, float2* tv, float* ow)
+{
+    const float xs = X + f.x, ys = Y + f.y;
+    tv[idx] = sample2(tvf, H, W, xs, ys);
+    ow[idx] = sample1(other, H, W, xs, ys);
+}
+
+__global__ void __launch_bounds__(256) k_warp_prep(float2* __restrict__ tv, float* __restrict__ ow,
+                                                   const float2* __restrict__ flow, const float2* __restrict__ tvf,
+                                                   const float* __restrict__ other, int H, int W)
+{
+    const int X = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (X >= W || Y >= H) return;
+    const int idx = Y * W + X;
+    warp_prep_pixel(X, Y, idx, flow[idx], tvf, other, H, W, tv, ow);
+}
+
+// medianFilter + computeOpticalFlowWithClamp (+ u = u_before_solver), and the
+// next warp's findWarpingVector / warpingImage at the updated flow
+// (Python/vfs.py:518-521, then :480,483).  u_ws holds the warp-start u and
+// receives the clamped u; u_th is the thresholded u of the primal-dual loop.
+template <int KM>
+__global__ void __launch_bounds__(256) k_median_clamp_prep(float* __restrict__ u_ws, float2* __restrict__ flow,
+                                                           float2* __restrict__ tv, float* __restrict__ ow,
+                                                           const float* __restrict__ u_th,
+                                                           const uint8_t* __restrict__ mask,
+                                                           const float2* __restrict__ tvf,
+                                                           const float* __restrict__ other, int H, int W, float limit,
+                                                           int prep)
+{
+    const int X = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (X >= W || Y >= H) return;
+    const int idx = Y * W + X;
+    float2 f = flow[idx];
+    if (mask[idx] != 0) {
+        constexpr int N = KM * KM;
+        constexpr int M = N / 2;
+        float v[N];
+#pragma unroll
+        for (int j = 0; j < KM; ++j) {
+            const int yy = clamp_idx(Y + j - KM / 2, H);
+#pragma unroll
+            for (int i = 0; i < KM; ++i) v[j * KM + i] = u_th[yy * W + clamp_idx(X + i - KM / 2, W)];
+        }
+#pragma unroll
+        for (int j = 0; j <= M; ++j) {
+#pragma unroll
+            for (int i = j + 1; i < N; ++i) {
+                const float lo = fminf(v[j], v[i]);
+                const float hi = fmaxf(v[j], v[i]);
+                v[j] = lo;
+                v[i] = hi;
+            }
+        }
+        const float base = u_ws[idx];
+        const float delta = fminf(v[M] - base, limit);
+        u_ws[idx] = base + delta;
+        const float2 t = tv[idx];
+        f.x = fmaf(delta, t.x, f.x);
+        f.y = fmaf(delta, t.y, f.y);
+        flow[idx] = f;
+    }
+    if (prep) warp_prep_pixel(X, Y, idx, f, tvf, other, H, W, tv, ow);
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+using PDFn = void (*)(PDLevel, int, PDSched, PDConsts);
+using PDBFn = void (*)(const PDLevel*, const int*, int, int, PDSched, PDConsts);
+
+struct Variant {
+    int tx, ty, k, nt;
+    PDFn single;
+    PDBFn batched;
+};
+
+#define VFS_PD_VARIANT(TX, TY, K, NT) \
+    {TX, TY, K, NT, k_pd_block<TX, TY, K, NT>, k_pd_block_batched<TX, TY, K, NT>}
+
+// Tile shapes and iteration counts available to the autotuner.  LDS use is
+// 36 bytes per pixel of the extended (TX + 3K) x (TY + 3K) region.
+const Variant kVariants[] = {
+    VFS_PD_VARIANT(16, 16, 1, 256),  VFS_PD_VARIANT(16, 16, 2, 512),  VFS_PD_VARIANT(16, 16, 4, 1024),
+    VFS_PD_VARIANT(16, 16, 8, 1024), VFS_PD_VARIANT(32, 32, 1, 1024), VFS_PD_VARIANT(32, 32, 2, 1024),
+    VFS_PD_VARIANT(32, 32, 4, 1024), VFS_PD_VARIANT(64, 32, 2, 1024),
+    VFS_PD_VARIANT(64, 16, 2, 1024), VFS_PD_VARIANT(64, 16, 4, 1024),
+    VFS_PD_VARIANT(32, 16, 4, 1024), VFS_PD_VARIANT(32, 16, 8, 1024), VFS_PD_VARIANT(8, 8, 8, 512),
+    VFS_PD_VARIANT(16, 16, 2, 256),  VFS_PD_VARIANT(32, 32, 2, 256),  VFS_PD_VARIANT(32, 32, 4, 256),
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+const Variant* find_variant(int tx, int ty, int k)
+{
+    for (int i = 0; i < kNumVariants; ++i)
+        if (kVariants[i].tx == tx && kVariants[i].ty == ty && kVariants[i].k == k) return &kVariants[i];
+    return nullptr;
+}
+
+PDConsts make_consts(const vfs_params* prm)
+{
+    PDConsts c;
+    c.alpha0 = prm->alpha0;
+    c.alpha1 = prm->alpha1;
+    c.eta_p = prm->eta_p;
+    c.eta_q = prm->eta_q;
+    c.lambda = prm->lambda;
+    return c;
+}
+
+int launch_pd_single(hipStream_t s, const Variant* v, PDLevel L, int k_count, const PDSched& sc, const PDConsts& c)
+{
+    L.tiles_x = (L.W + v->tx - 1) / v->tx;
+    L.tiles_y = (L.H + v->ty - 1) / v->ty;
+    hipLaunchKernelGGL(v->single, dim3(L.tiles_x * L.tiles_y), dim3(v->nt), 0, s, L, k_count, sc, c);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : static_cast<int>(e);
+}
+
+#define VFS_TRY(expr)              \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != 0) return rc_;  \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int vfs_pd_variants(int* out, int max_variants)
+{
+    if (out == nullptr) return kNumVariants;
+    const int n = max_variants < kNumVariants ? max_variants : kNumVariants;
+    for (int i = 0; i < n; ++i) {
+        out[4 * i + 0] = kVariants[i].tx;
+        out[4 * i + 1] = kVariants[i].ty;
+        out[4 * i + 2] = kVariants[i].k;
+        out[4 * i + 3] = kVariants[i].nt;
+    }
+    return n;
+}
+
+int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* abc, const float* eta,
+                  const float* deriv, const float* u_in, const float* ub_in, const float* vb_in, const float* p_in,
+                  const float* q_in, float* u_out, float* ub_out, float* vb_out, float* p_out, float* q_out,
+                  int first, const float* taus, const float* sigmas, const float* mus, int n_iter, int tile_x,
+                  int tile_y, int k, const vfs_params* prm)
+{
+    VFS_CHECK_DIMS(H, W);
+    const void* ptrs[] = {mask, abc, eta, deriv, u_in, ub_in, u_out, ub_out, vb_out, p_out, q_out, taus, sigmas, mus,
+                          prm};
+    for (const void* p : ptrs)
+        if (p == nullptr) return VFS_EINVAL;
+    if (!first && (vb_in == nullptr || p_in == nullptr || q_in == nullptr)) return VFS_EINVAL;
+    if (n_iter < 1 || n_iter > k) return VFS_EINVAL;
+    const Variant* v = find_variant(tile_x, tile_y, k);
+    if (v == nullptr) return VFS_EINVAL;
+    PDLevel L;
+    L.H = H;
+    L.W = W;
+    L.first = first;
+    L.mask = mask;
+    L.abc = abc;
+    L.eta = eta;
+    L.deriv = deriv;
+    L.u_in = u_in;
+    L.ub_in = ub_in;
+    L.vb_in = vb_in;
+    L.p_in = p_in;
+    L.q_in = q_in;
+    L.u_out = u_out;
+    L.ub_out = ub_out;
+    L.vb_out = vb_out;
+    L.p_out = p_out;
+    L.q_out = q_out;
+    PDSched sc;
+    for (int i = 0; i < KMAX; ++i) {
+        sc.tau[i] = i < n_iter ? taus[i] : 0.0f;
+        sc.sigma[i] = i < n_iter ? sigmas[i] : 0.0f;
+        sc.mu[i] = i < n_iter ? mus[i] : 0.0f;
+    }
+    return launch_pd_single(S(stream), v, L, n_iter, sc, make_consts(prm));
+}
+
+int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* ws, const vfs_params* prm,
+                            const float* taus, const float* sigmas, const float* mus, int n_iter, int n_warp,
+                            int tile_x, int tile_y, int k)
+{
+    if (lv == nullptr || ws == nullptr || prm == nullptr) return VFS_EINVAL;
+    VFS_CHECK_DIMS(lv->H, lv->W);
+    const void* ptrs[] = {lv->ref, lv->other, lv->tvf, lv->mask, lv->flow, lv->u, lv->u_bar, lv->ref_smoothed,
+                          lv->abc, lv->eta, lv->tv, lv->warped, lv->deriv, lv->p, lv->q, lv->v_bar,
+                          ws->u_th0, ws->u_th1, ws->u_bar2, ws->v_bar2, ws->p2, ws->q2};
+    for (const void* p : ptrs)
+        if (p == nullptr) return VFS_EINVAL;
+    if (n_iter < 0 || n_warp < 0 || (n_iter > 0 && (taus == nullptr || sigmas == nullptr || mus == nullptr)))
+        return VFS_EINVAL;
+    if (prm->median_k != 3 && prm->median_k != 5) return VFS_EINVAL;
+    const Variant* v = find_variant(tile_x, tile_y, k);
+    if (v == nullptr || k > KMAX) return VFS_EINVAL;
+    hipStream_t s = S(stream);
+    const int H = lv->H, W = lv->W;
+    VFS_TRY(vfs_gaussian_masked(stream, lv->ref_smoothed, lv->ref, lv->mask, H, W));
+    VFS_TRY(vfs_compute_tensor_eta(stream, lv->abc, lv->eta, lv->ref_smoothed, lv->mask, H, W, prm));
+    if (n_warp == 0) return 0;
+    hipLaunchKernelGGL(k_warp_prep, grid2d(H, W), block2d(), 0, s, reinterpret_cast<float2*>(lv->tv), lv->warped,
+                       reinterpret_cast<const float2*>(lv->flow), reinterpret_cast<const float2*>(lv->tvf),
+                       lv->other, H, W);
+    // state set 0 / 1 (ping-pong): thresholded u, u_bar, v_bar, p, q
+    float* su[2] = {ws->u_th0, ws->u_th1};
+    float* sub[2] = {lv->u_bar, ws->u_bar2};
+    float* svb[2] = {lv->v_bar, ws->v_bar2};
+    float* sp[2] = {lv->p, ws->p2};
+    float* sq[2] = {lv->q, ws->q2};
+    int cur = 0;  // set holding u_bar
+    const PDConsts cst = make_consts(prm);
+    for (int w = 0; w < n_warp; ++w) {
+        VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+        int done = 0;
+        bool first = true;
+        while (done < n_iter) {
+            const int kc = (n_iter - done) < k ? (n_iter - done) : k;
+            PDLevel L;
+            L.H = H;
+            L.W = W;
+            L.first = first ? 1 : 0;
+            L.mask = lv->mask;
+            L.abc = lv->abc;
+            L.eta = lv->eta;
+            L.deriv = lv->deriv;
+            L.u_in = first ? lv->u : su[cur];
+            L.ub_in = sub[cur];
+            L.vb_in = svb[cur];
+            L.p_in = sp[cur];
+            L.q_in = sq[cur];
+            L.u_out = su[1 - cur];
+            L.ub_out = sub[1 - cur];
+            L.vb_out = svb[1 - cur];
+            L.p_out = sp[1 - cur];
+            L.q_out = sq[1 - cur];
+            PDSched sc;
+            for (int i = 0; i < KMAX; ++i) {
+                sc.tau[i] = i < kc ? taus[done + i] : 0.0f;
+                sc.sigma[i] = i < kc ? sigmas[done + i] : 0.0f;
+                sc.mu[i] = i < kc ? mus[done + i] : 0.0f;
+            }
+            VFS_TRY(launch_pd_single(s, v, L, kc, sc, cst));
+            cur = 1 - cur;
+            done += kc;
+            first = false;
+        }
+        // with n_iter == 0 the thresholded u is the warp-start u
+        const float* uth = n_iter > 0 ? su[cur] : lv->u;
+        const int prep = (w + 1 < n_warp) ? 1 : 0;
+        if (prm->median_k == 5)
+            hipLaunchKernelGGL(k_median_clamp_prep<5>, grid2d(H, W), block2d(), 0, s, lv->u,
+                               reinterpret_cast<float2*>(lv->flow), reinterpret_cast<float2*>(lv->tv), lv->warped,
+                               uth, lv->mask, reinterpret_cast<const float2*>(lv->tvf), lv->other, H, W,
+                               prm->limit_range, prep);
+        else
+            hipLaunchKernelGGL(k_median_clamp_prep<3>, grid2d(H, W), block2d(), 0, s, lv->u,
+                               reinterpret_cast<float2*>(lv->flow), reinterpret_cast<float2*>(lv->tv), lv->warped,
+                               uth, lv->mask, reinterpret_cast<const float2*>(lv->tvf), lv->other, H, W,
+                               prm->limit_range, prep);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return static_cast<int>(e);
+    }
+    if (cur == 1) VFS_TRY(vfs_copy(stream, lv->u_bar, ws->u_bar2, static_cast<int64_t>(H) * W));
+    return 0;
+}
+
+}  // extern "C"

@@ -17,7 +17,13 @@ attribute ``solver`` selects how it is driven:
   recorded once into a HIP graph and replayed per call;
 * ``"eager"``: the same launches enqueued from C++ on each call;
 * ``"python"``: the reference's host loop literally, one Python call per
-  kernel through the per-kernel methods below.
+  kernel through the per-kernel methods below;
+* ``"blocked"`` / ``"blocked_eager"``: the same solve with the primal-dual loop
+  temporally blocked (k iterations per launch on LDS-exchanged register tiles,
+  see csrc/vfs_pd_block.hip) and the per-warp kernels fused, recorded or
+  enqueued per call.  The tile shape and k per pyramid level come from
+  ``pd_tiles`` (a dict {level: (tile_x, tile_y, k)}) or from the autotuned
+  table (autotune.py).
 """
 
 import ctypes
@@ -55,6 +61,7 @@ class VFS:
         self.param = param
         self.gpu_module = None
         self.solver = "graph"
+        self.pd_tiles = None
         self.median_filter_kernel_size = 5
         self._plan = None
         self._plan_key = None
@@ -290,12 +297,12 @@ class VFS:
         self.setup_module()
         if self.solver == "python":
             return self._solve_stereo_python(img_ref, img_other)
-        if self.solver not in ("graph", "eager"):
+        if self.solver not in ("graph", "eager", "blocked", "blocked_eager"):
             raise ValueError(f"unknown solver {self.solver!r}")
         b = self._buffers()
         b["img_ref"].copy_(_u8_device(img_ref))
         b["img_other"].copy_(_u8_device(img_other))
-        if self.solver == "eager":
+        if self.solver in ("eager", "blocked_eager"):
             self._enqueue_solve(H.stream(), b)
         else:
             key = self._key(b)
@@ -334,7 +341,18 @@ class VFS:
                 float(p.beta), float(p.gamma), float(p.alpha0), float(p.alpha1), float(p.Lambda),
                 float(p.limitRange), float(p.fScale), self.median_filter_kernel_size,
                 tuple(m.data_ptr() for m in self.masks), tuple(t.data_ptr() for t in self.translations),
-                self.calibration.data_ptr(), id(b))
+                self.calibration.data_ptr(), id(b), self.solver, tuple(self._tiles()))
+
+    def _tiles(self):
+        """(tile_x, tile_y, k) of the blocked primal-dual loop for every pyramid level."""
+        import autotune
+        out = []
+        for lvl, (h, w) in enumerate(self.sizes):
+            if self.pd_tiles is not None and lvl in self.pd_tiles:
+                out.append(tuple(self.pd_tiles[lvl]))
+            else:
+                out.append(autotune.tile_for(h, w, int(self.param.nSolverIters)))
+        return out
 
     def _buffers(self):
         """Persistent device buffers of the solve, sized by self.sizes (reused across calls)."""
@@ -355,6 +373,7 @@ class VFS:
             e = lambda *c: _empty((*sz, *c))  # noqa: E731
             b["levels"].append(dict(flow=e(2), u=e(), u_bar=e(), ref_smoothed=e(), abc=e(3), eta=e(3), tv=e(2),
                                     warped=e(), deriv=e(2), p=e(2), q=e(4), tp=e(2), v_bar=e(2), u0=e(), uf=e()))
+            b.setdefault("ws", []).append(dict(u_th0=e(), u_th1=e(), u_bar2=e(), v_bar2=e(2), p2=e(2), q2=e(4)))
         b["result"] = _empty((h0, w0, 2))
         self._bufs = b
         return b
@@ -365,6 +384,8 @@ class VFS:
         prm = self._prm()
         sizes = self.sizes
         nL = len(sizes)
+        blocked = self.solver in ("blocked", "blocked_eager")
+        tiles = self._tiles() if blocked else None
         for d, src in ((b["ref0"], b["img_ref"]), (b["other0"], b["img_other"])):
             H.call("vfs_cast_and_scale", s, H.ptr(d), H.ptr(src), d.shape[0], d.shape[1])
         self._image_pyramid_into(s, b["refs"])
@@ -394,8 +415,16 @@ class VFS:
             lv.mask = self.masks[k - 1].data_ptr()
             for name, t in L.items():
                 setattr(lv, name, t.data_ptr())
-            H.call("vfs_solve_level_plain", s, ctypes.byref(lv), ctypes.byref(prm), *sched, int(p.nSolverIters),
-                   int(p.nWarpIters))
+            if blocked:
+                ws = H.VfsPdWs()
+                for name, t in b["ws"][k - 1].items():
+                    setattr(ws, name, t.data_ptr())
+                tx, ty, kk = tiles[k - 1]
+                H.call("vfs_solve_level_blocked", s, ctypes.byref(lv), ctypes.byref(ws), ctypes.byref(prm), *sched,
+                       int(p.nSolverIters), int(p.nWarpIters), tx, ty, kk)
+            else:
+                H.call("vfs_solve_level_plain", s, ctypes.byref(lv), ctypes.byref(prm), *sched,
+                       int(p.nSolverIters), int(p.nWarpIters))
         L0 = b["levels"][0]
         H.call("vfs_copy", s, H.ptr(b["result"]), H.ptr(L0["flow"]), L0["flow"].numel())
         H.call("vfs_zero_masked", s, H.ptr(b["result"]), H.ptr(self.masks[0]), sizes[0][0], sizes[0][1], 2)

@@ -44,6 +44,12 @@ class VfsLevel(ctypes.Structure):
     ]
 
 
+class VfsPdWs(ctypes.Structure):
+    """struct vfs_pd_ws (include/vfs_hip.h)."""
+
+    _fields_ = [(n, c_p) for n in ("u_th0", "u_th1", "u_bar2", "v_bar2", "p2", "q2")]
+
+
 # name -> argtypes (restype is int for all but vfs_version)
 _SIGS = {
     "vfs_cast_and_scale": [c_p, c_p, c_p, c_i, c_i],
@@ -75,6 +81,10 @@ _SIGS = {
     "vfs_solve_level_plain": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i],
     "vfs_plan_create_level_plain": [ctypes.POINTER(c_p), ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams),
                                     c_p, c_p, c_p, c_i, c_i],
+    "vfs_pd_variants": [c_p, c_i],
+    "vfs_pd_blocks": [c_p, c_i, c_i] + [c_p] * 14 + [c_i, c_p, c_p, c_p, c_i, c_i, c_i, c_i, ctypes.POINTER(VfsParams)],
+    "vfs_solve_level_blocked": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsPdWs), ctypes.POINTER(VfsParams),
+                                c_p, c_p, c_p, c_i, c_i, c_i, c_i, c_i],
     "vfs_capture_begin": [ctypes.POINTER(c_p)],
     "vfs_capture_end": [c_p, ctypes.POINTER(c_p)],
     "vfs_plan_launch": [c_p, c_p],
@@ -150,6 +160,15 @@ def make_params(param, median_k=5):
     p.a004 = float(np.float32(4 * float(param.alpha0) * float(param.alpha0)))
     p.median_k = int(median_k)
     return p
+
+
+def pd_variants():
+    """(tile_x, tile_y, k, threads) of every blocked primal-dual variant in the library."""
+    L = lib()
+    n = L.vfs_pd_variants(None, 0)
+    buf = (ctypes.c_int * (4 * n))()
+    n = L.vfs_pd_variants(ctypes.cast(buf, c_p), n)
+    return [tuple(buf[4 * i:4 * i + 4]) for i in range(n)]
 
 
 def pd_schedule(n_iter, time_step_lambda):
