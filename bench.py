@@ -154,6 +154,94 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
                 achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
 
 
+def run_batch(a):
+    """64 pairs of uneven size (16 each of 512^2, 800^2, 1024^2, 1280x960) through VFSBatch."""
+    import torch
+    import synthetic as S
+    import vfs as V
+    import vfs_batch as B
+    import vfs_params as P
+    import vector_field as VF
+    import equidistant_cam as E
+    spec = S.BATCH_64
+    log(f"[bench] batch: generating {a.pairs} pairs on the GPU")
+    pairs = S.batch_pairs(spec, device="cuda", n=a.pairs)
+    sizes = [p["img_ref"].shape for p in pairs]
+    prm = P.vfs_params()
+    S.apply_workload_params(prm, spec)
+    batch = B.VFSBatch(prm, sizes)
+    cams = []
+    for i, p in enumerate(pairs):
+        cam = E.EquidistantCam(p["cam"])
+        cam.setup_module()
+        cams.append(cam)
+        tvf, cvf = VF.get_vector_fields(VF.get_grid_zero(sizes[i]), cam, cam, p["R"], p["t"], 1, VF.unit_normalizer)
+        batch.setup_pair(i, p["mask"], S.sanitize_field(tvf), cvf)
+    refs = [torch.as_tensor(p["img_ref"], device="cuda") for p in pairs]
+    others = [torch.as_tensor(p["img_other"], device="cuda") for p in pairs]
+    ts = [p["t"] for p in pairs]
+
+    def step():
+        batch.solve(refs, others)
+        return batch.triangulate(cams, ts)
+
+    log("[bench] batch: warmup (records the graph)")
+    for _ in range(max(1, a.warmup)):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] batch: timing {a.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    px = sum(sum(h * w for h, w in v.sizes) for v in batch.vfs)
+    px_iter = px * spec["nWarpIters"] * spec["nSolverIters"]
+    bytes_batch = sum(sum(h * w * (spec["nWarpIters"] * (spec["nSolverIters"] * B_ITER + B_WARP) + B_LEVEL)
+                          for h, w in v.sizes) for v in batch.vfs)
+    value = len(pairs) * a.steps / dt
+    plain = None
+    if not a.no_compare:
+        # the per-kernel path, one pair at a time: one pair of each size class timed, scaled by its count
+        per_size = {}
+        for i, s in enumerate(sizes):
+            if s in per_size:
+                continue
+            v = V.VFS(batch.vfs[i].param)
+            v.solver = "graph"
+            v.setup_module()
+            v.compute_pyramid_sizes()
+            v.masks, v.masks_to = batch.vfs[i].masks, batch.vfs[i].masks_to
+            v.translations, v.translations_to = batch.vfs[i].translations, batch.vfs[i].translations_to
+            v.calibration = batch.vfs[i].calibration
+            v.solve_stereo(refs[i], others[i])
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(2):
+                v.solve_stereo(refs[i], others[i])
+                v.triangulate_flow(cams[i], ts[i])
+            torch.cuda.synchronize()
+            per_size[s] = (time.perf_counter() - t1) / 2
+            log(f"[bench] per-kernel path {s}: {1e3 * per_size[s]:.1f} ms/pair")
+        tot = sum(per_size[s] for s in sizes)
+        plain = dict(value=round(len(pairs) / tot, 3), ms_per_batch=round(1e3 * tot, 1),
+                     speedup_of_reported=round(tot / (dt / a.steps), 2))
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "depth maps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched_64", "pairs": len(pairs),
+                   "sizes": sorted({f"{w}x{h}" for h, w in sizes}), "levels": spec["nLevel"],
+                   "warps_per_level": spec["nWarpIters"], "pd_iters_per_warp": spec["nSolverIters"],
+                   "solver": "batched blocked (one graph per batch)", "pd_tiles": [list(t) for t in batch._tiles()]},
+        "plain_per_kernel": plain,
+        "mpx_iter_per_s": round(px_iter * a.steps / dt / 1e6, 1),
+        "solver_effective_gbs": round(bytes_batch * a.steps / dt / 1e9, 1),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_oracle_baseline(ctx, name):
     """float64 NumPy oracle (1 thread) on the full pyramid with 1 of nWarpIters warps per level."""
     from oracle import vfs_oracle as O
@@ -193,7 +281,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain_full_800")
+    ap.add_argument("--workload", default="plain_full_800", choices=["plain_full_800", "plain_small_256", "batched_64"])
+    ap.add_argument("--pairs", type=int, default=64, help="batched_64: number of pairs (first n of the 64)")
     ap.add_argument("--solver", default="blocked", choices=["graph", "eager", "blocked", "blocked_eager"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain per-kernel path beside it")
     ap.add_argument("--seed", type=int, default=0)
@@ -209,6 +298,9 @@ def main():
     import torch
     import synthetic as S
     torch.cuda.set_device(0)
+    if a.workload == "batched_64":
+        run_batch(a)
+        return
     w = S.WORKLOADS[a.workload]
     log(f"[bench] workload {a.workload}: generating the pair and vector fields")
     ctx = setup_solver(a.workload, a.seed, a.solver)

@@ -24,6 +24,7 @@
 #ifndef VFS_HIP_H_
 #define VFS_HIP_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -266,6 +267,28 @@ int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* 
                             const vfs_params* prm, const float* taus, const float* sigmas,
                             const float* mus, int n_iter, int n_warp, int tile_x, int tile_y,
                             int k);
+
+/* ---- batches of pairs (no reference counterpart: the reference solves one pair
+ * per VFS object) ---------------------------------------------------------------- */
+
+/* Bytes of device workspace that one level of an n_pairs batch needs for its
+ * descriptor tables. */
+size_t vfs_batch_workspace_bytes(int n_pairs);
+
+/* Writes the descriptor tables of one pyramid level of a batch (lv[i], ws[i] for
+ * pair i, any sizes) into d_workspace with a synchronous copy.  Call it before
+ * recording a graph that uses the level; the tables hold the buffer pointers. */
+int vfs_batch_prepare_level(void* d_workspace, size_t workspace_bytes, const vfs_level* lv,
+                            const vfs_pd_ws* ws, int n_pairs, int tile_x, int tile_y, int k);
+
+/* vfs_solve_level_blocked for every pair of the batch at once: per warp, one
+ * batched derivatives launch, ceil(n_iter / k) batched blocked launches over the
+ * tiles of all pairs, and one batched median / clamp / next-warp kernel.
+ * d_workspace must have been prepared with the same lv, ws, tile_x, tile_y, k. */
+int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const vfs_level* lv,
+                                  const vfs_pd_ws* ws, int n_pairs, const vfs_params* prm,
+                                  const float* taus, const float* sigmas, const float* mus,
+                                  int n_iter, int n_warp, int tile_x, int tile_y, int k);
 
 /* Library version string (static storage). */
 const char* vfs_version(void);

@@ -1,0 +1,80 @@
+"""Batched solve of pairs with different sizes against single-pair solves.
+
+The batch runs every pair's level through the same batched kernels (one
+derivatives launch, the blocked primal-dual launches over all pairs' tiles, one
+median/clamp launch per warp); each pair's flow must equal its own single-pair
+blocked solve with the same tiles, bit for bit.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import vfs_oracle as O
+from conftest import flow_diff_stats
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(96, 128), (128, 128), (80, 112)]
+TILES = {0: (16, 16, 4), 1: (16, 16, 4)}
+
+
+@pytest.fixture(scope="module")
+def T():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _params():
+    import vfs_params as P
+    prm = P.vfs_params()
+    prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = 2, 2.0, 3, 10
+    prm.Lambda, prm.alpha0, prm.alpha1, prm.beta, prm.gamma = 50.0, 5.0, 1.0, 10.0, 0.5
+    return prm
+
+
+def _pairs():
+    import synthetic as S
+    out = []
+    for i, (h, w) in enumerate(SIZES):
+        pr = S.make_pair(h, w, seed=100 + i, t=(0.2, 0.0, 0.02), rot_deg=1.5, rot_random=True, n_levels=2)
+        cam = pr["cam"]
+        tvf, cvf = O.get_vector_fields(O.get_grid_zero((h, w)), cam, cam, pr["R"], pr["t"], 1)
+        out.append(dict(pair=pr, tvf=S.sanitize_field(tvf), cvf=cvf.astype(np.float32)))
+    return out
+
+
+def test_batch_equals_single_pair_solves(T):
+    import copy
+    import vfs as V
+    import vfs_batch as B
+    pairs = _pairs()
+    prm = _params()
+    batch = B.VFSBatch(prm, SIZES)
+    batch.pd_tiles = TILES
+    for i, c in enumerate(pairs):
+        batch.setup_pair(i, c["pair"]["mask"], c["tvf"], c["cvf"])
+    flows = batch.solve([c["pair"]["img_ref"] for c in pairs], [c["pair"]["img_other"] for c in pairs])
+    flows = [f.cpu().numpy() for f in flows]
+    again = [f.cpu().numpy() for f in batch.solve([c["pair"]["img_ref"] for c in pairs],
+                                                  [c["pair"]["img_other"] for c in pairs])]
+    for (h, w), c, f, f2 in zip(SIZES, pairs, flows, again):
+        np.testing.assert_array_equal(f, f2)
+        p = copy.copy(prm)
+        p.height, p.width = h, w
+        v = V.VFS(p)
+        v.solver = "blocked"
+        v.pd_tiles = TILES
+        v.setup_module()
+        v.compute_pyramid_sizes()
+        v.create_mask_pyramid(c["pair"]["mask"])
+        v.create_vector_field_pyramid(c["tvf"], c["cvf"])
+        single = v.solve_stereo(c["pair"]["img_ref"], c["pair"]["img_other"]).cpu().numpy()
+        np.testing.assert_array_equal(f, single)
+        oprm = O.OracleParams(h, w, nLevel=2, fScale=2.0, nWarpIters=3, nSolverIters=10, Lambda=50.0, alpha0=5.0,
+                              alpha1=1.0, beta=10.0, gamma=0.5)
+        fo = O.solve_stereo(c["pair"]["img_ref"], c["pair"]["img_other"], c["pair"]["mask"], c["tvf"], c["cvf"], oprm)
+        st = flow_diff_stats(f, fo, c["pair"]["mask"])
+        print((h, w), "batched vs oracle:", st)
+        assert st["median"] <= 1e-4

@@ -81,6 +81,33 @@ __device__ __forceinline__ float2 sample2(const float2* __restrict__ img, int H,
     return make_float2(lerp2(a.x, b.x, c.x, d.x, t.ax, t.ay), lerp2(a.y, b.y, c.y, d.y, t.ax, t.ay));
 }
 
+// computeDerivatives at one pixel, Python/vfs.cu:120-142: Iu = 5-tap directional
+// derivative of ref and other along the unit translation direction, Iz = other - ref.
+__device__ __forceinline__ float2 derivative_pixel(int X, int Y, int idx, const float* __restrict__ ref,
+                                                   const float* __restrict__ other, const float2* __restrict__ tvf,
+                                                   int H, int W)
+{
+    const float2 vec = tvf[idx];
+    const float nrm = hypotf(vec.x, vec.y);
+    const float ex = vec.x / nrm, ey = vec.y / nrm;
+    const float x = static_cast<float>(X), y = static_cast<float>(Y);
+    const float xm2 = fmaf(-2.0f, ex, x), ym2 = fmaf(-2.0f, ey, y);
+    const float xm1 = x - ex, ym1 = y - ey;
+    const float xp1 = x + ex, yp1 = y + ey;
+    const float xp2 = fmaf(2.0f, ex, x), yp2 = fmaf(2.0f, ey, y);
+    float v0 = sample1(ref, H, W, xm2, ym2);
+    v0 -= sample1(ref, H, W, xm1, ym1) * 8;
+    v0 += sample1(ref, H, W, xp1, yp1) * 8;
+    v0 -= sample1(ref, H, W, xp2, yp2);
+    float v1 = sample1(other, H, W, xm2, ym2);
+    v1 -= sample1(other, H, W, xm1, ym1) * 8;
+    v1 += sample1(other, H, W, xp1, yp1) * 8;
+    v1 -= sample1(other, H, W, xp2, yp2);
+    const float v2 = ref[idx];
+    const float v3 = other[idx];
+    return make_float2((v0 + v1) / 24, v3 - v2);
+}
+
 // Kernel-side copy of the solver constants.
 struct KParams {
     float beta, gamma, alpha0, alpha1, eta_p, eta_q, lambda, limit_range;

@@ -175,25 +175,7 @@ __global__ void __launch_bounds__(256) k_derivatives(float2* __restrict__ out, c
                                                      int H, int W)
 {
     VFS_PIXEL(H, W)
-    const float2 vec = tvf[idx];
-    const float nrm = hypotf(vec.x, vec.y);
-    const float ex = vec.x / nrm, ey = vec.y / nrm;
-    const float x = static_cast<float>(X), y = static_cast<float>(Y);
-    const float xm2 = fmaf(-2.0f, ex, x), ym2 = fmaf(-2.0f, ey, y);
-    const float xm1 = x - ex, ym1 = y - ey;
-    const float xp1 = x + ex, yp1 = y + ey;
-    const float xp2 = fmaf(2.0f, ex, x), yp2 = fmaf(2.0f, ey, y);
-    float v0 = sample1(ref, H, W, xm2, ym2);
-    v0 -= sample1(ref, H, W, xm1, ym1) * 8;
-    v0 += sample1(ref, H, W, xp1, yp1) * 8;
-    v0 -= sample1(ref, H, W, xp2, yp2);
-    float v1 = sample1(other, H, W, xm2, ym2);
-    v1 -= sample1(other, H, W, xm1, ym1) * 8;
-    v1 += sample1(other, H, W, xp1, yp1) * 8;
-    v1 -= sample1(other, H, W, xp2, yp2);
-    const float v2 = ref[idx];
-    const float v3 = other[idx];
-    out[idx] = make_float2((v0 + v1) / 24, v3 - v2);
+    out[idx] = derivative_pixel(X, Y, idx, ref, other, tvf, H, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -220,7 +202,8 @@ __global__ void __launch_bounds__(256) k_update_dual(float2* __restrict__ p, flo
     float2 pn, tpn;
     float4 qn;
     dual_pixel(u[idx], v[idx], u[ix], v[ix], fx, u[iy], v[iy], fy, abc[3 * idx + 0], abc[3 * idx + 1],
-               abc[3 * idx + 2], p[idx], q[idx], sigma, alpha0, alpha1, eta_p, eta_q, pn, tpn, qn);
+               abc[3 * idx + 2], p[idx], q[idx], dual_scale(alpha1, sigma, eta_p), dual_scale(alpha0, sigma, eta_q),
+               pn, tpn, qn);
     p[idx] = pn;
     tp[idx] = tpn;
     q[idx] = qn;

@@ -10,13 +10,22 @@
 
 namespace vfs {
 
+// The two step sizes of the dual ascent, alpha1 sigma / eta_p and alpha0 sigma /
+// eta_q (Python/vfs.cu:387,397), as the reference forms them in fp32.
+__host__ __device__ __forceinline__ float dual_scale(float alpha, float sigma, float eta)
+{
+    return alpha * sigma / eta;
+}
+
 // Dual ascent with pointwise projection.  The caller selects the neighbour used
 // for each axis: (uX, vX) is the east neighbour when fwd_x (forward difference)
 // and the west neighbour otherwise; likewise (uY, vY) south / north.
+// scale_p / scale_q are dual_scale(alpha1, sigma, eta_p) / dual_scale(alpha0, sigma, eta_q).
+// The projection divides by max(norm, 1); division by 1 is exact, so it is
+// skipped when the norm does not exceed 1.
 __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2 vX, bool fwd_x, float uY, float2 vY,
-                                           bool fwd_y, float a, float b, float c, float2 pc, float4 qc, float sigma,
-                                           float alpha0, float alpha1, float eta_p, float eta_q, float2& pn,
-                                           float2& tpn, float4& qn)
+                                           bool fwd_y, float a, float b, float c, float2 pc, float4 qc, float scale_p,
+                                           float scale_q, float2& pn, float2& tpn, float4& qn)
 {
     float2 gU;
     float4 gV;
@@ -39,16 +48,24 @@ __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2
         gV.z = vc.x - vY.x;
     }
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
-    float scale = alpha1 * sigma / eta_p;
-    const float2 up = make_float2(fmaf(scale, tu.x, pc.x), fmaf(scale, tu.y, pc.y));
+    const float2 up = make_float2(fmaf(scale_p, tu.x, pc.x), fmaf(scale_p, tu.y, pc.y));
     float nrm = fmaxf(hypotf(up.x, up.y), 1.0f);
-    pn = make_float2(up.x / nrm, up.y / nrm);
+    pn = up;
+    if (nrm > 1.0f) {
+        pn.x = up.x / nrm;
+        pn.y = up.y / nrm;
+    }
     tpn = make_float2(fmaf(a, pn.x, c * pn.y), fmaf(c, pn.x, b * pn.y));
-    scale = alpha0 * sigma / eta_q;
-    const float4 uq = make_float4(fmaf(scale, gV.x, qc.x), fmaf(scale, gV.y, qc.y), fmaf(scale, gV.z, qc.z),
-                                  fmaf(scale, gV.w, qc.w));
+    const float4 uq = make_float4(fmaf(scale_q, gV.x, qc.x), fmaf(scale_q, gV.y, qc.y), fmaf(scale_q, gV.z, qc.z),
+                                  fmaf(scale_q, gV.w, qc.w));
     nrm = fmaxf(norm4df(uq.x, uq.y, uq.z, uq.w), 1.0f);
-    qn = make_float4(uq.x / nrm, uq.y / nrm, uq.z / nrm, uq.w / nrm);
+    qn = uq;
+    if (nrm > 1.0f) {
+        qn.x = uq.x / nrm;
+        qn.y = uq.y / nrm;
+        qn.z = uq.z / nrm;
+        qn.w = uq.w / nrm;
+    }
 }
 
 // Linearised-data-term threshold: the prox of lambda |Iu (x - u) + Iz| in the

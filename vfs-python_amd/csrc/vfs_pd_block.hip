@@ -16,6 +16,7 @@
 // The per-pixel arithmetic is the same inline code as the per-kernel path
 // (vfs_pd.h).
 #include <new>
+#include <vector>
 
 #include "vfs_device.h"
 #include "vfs_pd.h"
@@ -28,7 +29,8 @@ constexpr int KMAX = 8;
 
 struct PDSched {
     float tau[KMAX];
-    float sigma[KMAX];
+    float sp[KMAX];  // dual_scale(alpha1, sigma, eta_p) per iteration
+    float sq[KMAX];  // dual_scale(alpha0, sigma, eta_q) per iteration
     float mu[KMAX];
 };
 
@@ -145,8 +147,19 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
         }
     }
 
-    for (int k = 0; k < k_count; ++k) {
-        const float sigma = sc.sigma[k], tau = sc.tau[k], mu = sc.mu[k];
+    // A region with no pixel inside the mask ends every iteration with all state
+    // zero (masked pixels are zeroed by each of the three steps).
+    int any_active = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) any_active |= static_cast<int>(fl[j] & F_ACT);
+    const bool skip = __syncthreads_or(any_active) == 0 && k_count > 0;
+    if (skip) {
+#pragma unroll
+        for (int j = 0; j < S; ++j) ub[j] = vbx[j] = vby[j] = 0.0f;
+    }
+
+    for (int k = 0; k < (skip ? 0 : k_count); ++k) {
+        const float tau = sc.tau[k], mu = sc.mu[k], sp = sc.sp[k], sq = sc.sq[k];
         __syncthreads();  // sA (u_bar, v_bar) complete
         // dual step
 #pragma unroll
@@ -164,8 +177,8 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
                 float4 qn;
                 dual_pixel(ub[j], make_float2(vbx[j], vby[j]), sA[ix], make_float2(sA[E + ix], sA[2 * E + ix]), fx,
                            sA[iy], make_float2(sA[E + iy], sA[2 * E + iy]), fy, ca[j], cb[j], cc[j],
-                           make_float2(px[j], py[j]), make_float4(qx[j], qy[j], qz[j], qw[j]), sigma, cst.alpha0,
-                           cst.alpha1, cst.eta_p, cst.eta_q, pn, tpn, qn);
+                           make_float2(px[j], py[j]), make_float4(qx[j], qy[j], qz[j], qw[j]), sp, sq, pn,
+                           tpn, qn);
                 px[j] = pn.x;
                 py[j] = pn.y;
                 tpx = tpn.x;
@@ -303,17 +316,14 @@ __global__ void __launch_bounds__(256) k_warp_prep(float2* __restrict__ tv, floa
 // (Python/vfs.py:518-521, then :480,483).  u_ws holds the warp-start u and
 // receives the clamped u; u_th is the thresholded u of the primal-dual loop.
 template <int KM>
-__global__ void __launch_bounds__(256) k_median_clamp_prep(float* __restrict__ u_ws, float2* __restrict__ flow,
-                                                           float2* __restrict__ tv, float* __restrict__ ow,
-                                                           const float* __restrict__ u_th,
-                                                           const uint8_t* __restrict__ mask,
-                                                           const float2* __restrict__ tvf,
-                                                           const float* __restrict__ other, int H, int W, float limit,
-                                                           int prep)
+__device__ __forceinline__ void median_clamp_prep_pixel(int X, int Y, float* __restrict__ u_ws,
+                                                        float2* __restrict__ flow, float2* __restrict__ tv,
+                                                        float* __restrict__ ow, const float* __restrict__ u_th,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const float2* __restrict__ tvf,
+                                                        const float* __restrict__ other, int H, int W, float limit,
+                                                        int prep)
 {
-    const int X = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (X >= W || Y >= H) return;
     const int idx = Y * W + X;
     float2 f = flow[idx];
     if (mask[idx] != 0) {
@@ -345,6 +355,84 @@ __global__ void __launch_bounds__(256) k_median_clamp_prep(float* __restrict__ u
         flow[idx] = f;
     }
     if (prep) warp_prep_pixel(X, Y, idx, f, tvf, other, H, W, tv, ow);
+}
+
+template <int KM>
+__global__ void __launch_bounds__(256) k_median_clamp_prep(float* __restrict__ u_ws, float2* __restrict__ flow,
+                                                           float2* __restrict__ tv, float* __restrict__ ow,
+                                                           const float* __restrict__ u_th,
+                                                           const uint8_t* __restrict__ mask,
+                                                           const float2* __restrict__ tvf,
+                                                           const float* __restrict__ other, int H, int W, float limit,
+                                                           int prep)
+{
+    const int X = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (X >= W || Y >= H) return;
+    median_clamp_prep_pixel<KM>(X, Y, u_ws, flow, tv, ow, u_th, mask, tvf, other, H, W, limit, prep);
+}
+
+// ---------------------------------------------------------------------------
+// batched per-warp kernels: many pairs (of different sizes) in one launch
+// ---------------------------------------------------------------------------
+
+// One pair's level for the per-warp kernels.  Tiles are 64 x 4 pixels (one
+// wavefront per row segment); tiles_x = ceil(W / 64).
+struct WarpDesc {
+    int H, W, tiles_x;
+    const float* ref;
+    const float* other;
+    const float2* tvf;
+    const uint8_t* mask;
+    float2* flow;
+    float* u;
+    float2* tv;
+    float* warped;
+    float2* deriv;
+    const float* u_th;
+};
+
+__device__ __forceinline__ int find_item(const int* __restrict__ start, int n, int t)
+{
+    int lo = 0, hi = n - 1;  // last i with start[i] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_derivatives_batched(const WarpDesc* __restrict__ d,
+                                                             const int* __restrict__ start, int n)
+{
+    const int t = blockIdx.x;
+    const int i = find_item(start, n, t);
+    const WarpDesc& D = d[i];
+    const int lt = t - start[i];
+    const int X = (lt % D.tiles_x) * 64 + (threadIdx.x & 63);
+    const int Y = (lt / D.tiles_x) * 4 + (threadIdx.x >> 6);
+    if (X >= D.W || Y >= D.H) return;
+    const int idx = Y * D.W + X;
+    D.deriv[idx] = derivative_pixel(X, Y, idx, D.ref, D.warped, D.tvf, D.H, D.W);
+}
+
+template <int KM>
+__global__ void __launch_bounds__(256) k_median_clamp_prep_batched(const WarpDesc* __restrict__ d,
+                                                                   const int* __restrict__ start, int n, float limit,
+                                                                   int prep)
+{
+    const int t = blockIdx.x;
+    const int i = find_item(start, n, t);
+    const WarpDesc& D = d[i];
+    const int lt = t - start[i];
+    const int X = (lt % D.tiles_x) * 64 + (threadIdx.x & 63);
+    const int Y = (lt / D.tiles_x) * 4 + (threadIdx.x >> 6);
+    if (X >= D.W || Y >= D.H) return;
+    median_clamp_prep_pixel<KM>(X, Y, D.u, D.flow, D.tv, D.warped, D.u_th, D.mask, D.tvf, D.other, D.H, D.W, limit,
+                                prep);
 }
 
 // ---------------------------------------------------------------------------
@@ -393,6 +481,21 @@ PDConsts make_consts(const vfs_params* prm)
     c.eta_q = prm->eta_q;
     c.lambda = prm->lambda;
     return c;
+}
+
+// Schedule of one launch: tau, mu and the two dual step sizes (formed on the
+// host with the same fp32 operations as the per-kernel path forms them).
+PDSched make_sched(const vfs_params* prm, const float* taus, const float* sigmas, const float* mus, int n)
+{
+    PDSched sc;
+    for (int i = 0; i < KMAX; ++i) {
+        const bool in = i < n;
+        sc.tau[i] = in ? taus[i] : 0.0f;
+        sc.mu[i] = in ? mus[i] : 0.0f;
+        sc.sp[i] = in ? dual_scale(prm->alpha1, sigmas[i], prm->eta_p) : 0.0f;
+        sc.sq[i] = in ? dual_scale(prm->alpha0, sigmas[i], prm->eta_q) : 0.0f;
+    }
+    return sc;
 }
 
 int launch_pd_single(hipStream_t s, const Variant* v, PDLevel L, int k_count, const PDSched& sc, const PDConsts& c)
@@ -460,12 +563,7 @@ int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* 
     L.vb_out = vb_out;
     L.p_out = p_out;
     L.q_out = q_out;
-    PDSched sc;
-    for (int i = 0; i < KMAX; ++i) {
-        sc.tau[i] = i < n_iter ? taus[i] : 0.0f;
-        sc.sigma[i] = i < n_iter ? sigmas[i] : 0.0f;
-        sc.mu[i] = i < n_iter ? mus[i] : 0.0f;
-    }
+    const PDSched sc = make_sched(prm, taus, sigmas, mus, n_iter);
     return launch_pd_single(S(stream), v, L, n_iter, sc, make_consts(prm));
 }
 
@@ -525,12 +623,7 @@ int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* 
             L.vb_out = svb[1 - cur];
             L.p_out = sp[1 - cur];
             L.q_out = sq[1 - cur];
-            PDSched sc;
-            for (int i = 0; i < KMAX; ++i) {
-                sc.tau[i] = i < kc ? taus[done + i] : 0.0f;
-                sc.sigma[i] = i < kc ? sigmas[done + i] : 0.0f;
-                sc.mu[i] = i < kc ? mus[done + i] : 0.0f;
-            }
+            const PDSched sc = make_sched(prm, taus + done, sigmas + done, mus + done, kc);
             VFS_TRY(launch_pd_single(s, v, L, kc, sc, cst));
             cur = 1 - cur;
             done += kc;
@@ -553,6 +646,190 @@ int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* 
         if (e != hipSuccess) return static_cast<int>(e);
     }
     if (cur == 1) VFS_TRY(vfs_copy(stream, lv->u_bar, ws->u_bar2, static_cast<int64_t>(H) * W));
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches of pairs: one level of every pair per call
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct BatchLayout {
+    size_t pd, pd_start, wd, wd_start, total;
+};
+
+BatchLayout batch_layout(int n)
+{
+    BatchLayout l;
+    l.pd = 0;
+    l.pd_start = align_up(l.pd + 4 * static_cast<size_t>(n) * sizeof(PDLevel));
+    l.wd = align_up(l.pd_start + (static_cast<size_t>(n) + 1) * sizeof(int));
+    l.wd_start = align_up(l.wd + 2 * static_cast<size_t>(n) * sizeof(WarpDesc));
+    l.total = align_up(l.wd_start + (static_cast<size_t>(n) + 1) * sizeof(int));
+    return l;
+}
+
+int check_pair(const vfs_level* lv, const vfs_pd_ws* ws)
+{
+    if (lv == nullptr || ws == nullptr) return VFS_EINVAL;
+    VFS_CHECK_DIMS(lv->H, lv->W);
+    const void* ptrs[] = {lv->ref, lv->other, lv->tvf, lv->mask, lv->flow, lv->u, lv->u_bar, lv->ref_smoothed,
+                          lv->abc, lv->eta, lv->tv, lv->warped, lv->deriv, lv->p, lv->q, lv->v_bar,
+                          ws->u_th0, ws->u_th1, ws->u_bar2, ws->v_bar2, ws->p2, ws->q2};
+    for (const void* p : ptrs)
+        if (p == nullptr) return VFS_EINVAL;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t vfs_batch_workspace_bytes(int n_pairs)
+{
+    return n_pairs > 0 ? batch_layout(n_pairs).total : 0;
+}
+
+int vfs_batch_prepare_level(void* d_workspace, size_t workspace_bytes, const vfs_level* lv, const vfs_pd_ws* ws,
+                            int n_pairs, int tile_x, int tile_y, int k)
+{
+    if (d_workspace == nullptr || lv == nullptr || ws == nullptr || n_pairs <= 0) return VFS_EINVAL;
+    const BatchLayout lay = batch_layout(n_pairs);
+    if (workspace_bytes < lay.total) return VFS_EINVAL;
+    if (find_variant(tile_x, tile_y, k) == nullptr) return VFS_EINVAL;
+    std::vector<unsigned char> host(lay.total, 0);
+    PDLevel* pd = reinterpret_cast<PDLevel*>(host.data() + lay.pd);
+    int* pd_start = reinterpret_cast<int*>(host.data() + lay.pd_start);
+    WarpDesc* wd = reinterpret_cast<WarpDesc*>(host.data() + lay.wd);
+    int* wd_start = reinterpret_cast<int*>(host.data() + lay.wd_start);
+    int pt = 0, wt = 0;
+    for (int i = 0; i < n_pairs; ++i) {
+        VFS_TRY(check_pair(&lv[i], &ws[i]));
+        const vfs_level& l = lv[i];
+        const vfs_pd_ws& w = ws[i];
+        float* su[2] = {w.u_th0, w.u_th1};
+        float* sub[2] = {l.u_bar, w.u_bar2};
+        float* svb[2] = {l.v_bar, w.v_bar2};
+        float* sp[2] = {l.p, w.p2};
+        float* sq[2] = {l.q, w.q2};
+        // tables: 0 / 1 = first launch of a warp with u_bar in set 0 / 1; 2 / 3 = later launches reading set 0 / 1
+        for (int t = 0; t < 4; ++t) {
+            const int cur = t & 1, first = t < 2;
+            PDLevel& L = pd[t * n_pairs + i];
+            L.H = l.H;
+            L.W = l.W;
+            L.tiles_x = (l.W + tile_x - 1) / tile_x;
+            L.tiles_y = (l.H + tile_y - 1) / tile_y;
+            L.first = first;
+            L.mask = l.mask;
+            L.abc = l.abc;
+            L.eta = l.eta;
+            L.deriv = l.deriv;
+            L.u_in = first ? l.u : su[cur];
+            L.ub_in = sub[cur];
+            L.vb_in = svb[cur];
+            L.p_in = sp[cur];
+            L.q_in = sq[cur];
+            L.u_out = su[1 - cur];
+            L.ub_out = sub[1 - cur];
+            L.vb_out = svb[1 - cur];
+            L.p_out = sp[1 - cur];
+            L.q_out = sq[1 - cur];
+        }
+        pd_start[i] = pt;
+        pt += pd[i].tiles_x * pd[i].tiles_y;
+        for (int t = 0; t < 2; ++t) {
+            WarpDesc& D = wd[t * n_pairs + i];
+            D.H = l.H;
+            D.W = l.W;
+            D.tiles_x = (l.W + 63) / 64;
+            D.ref = l.ref;
+            D.other = l.other;
+            D.tvf = reinterpret_cast<const float2*>(l.tvf);
+            D.mask = l.mask;
+            D.flow = reinterpret_cast<float2*>(l.flow);
+            D.u = l.u;
+            D.tv = reinterpret_cast<float2*>(l.tv);
+            D.warped = l.warped;
+            D.deriv = reinterpret_cast<float2*>(l.deriv);
+            D.u_th = su[t];
+        }
+        wd_start[i] = wt;
+        wt += ((l.W + 63) / 64) * ((l.H + 3) / 4);
+    }
+    pd_start[n_pairs] = pt;
+    wd_start[n_pairs] = wt;
+    hipError_t e = hipMemcpy(d_workspace, host.data(), lay.total, hipMemcpyHostToDevice);
+    return e == hipSuccess ? 0 : static_cast<int>(e);
+}
+
+int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const vfs_level* lv, const vfs_pd_ws* ws,
+                                  int n_pairs, const vfs_params* prm, const float* taus, const float* sigmas,
+                                  const float* mus, int n_iter, int n_warp, int tile_x, int tile_y, int k)
+{
+    if (d_workspace == nullptr || lv == nullptr || ws == nullptr || prm == nullptr || n_pairs <= 0) return VFS_EINVAL;
+    if (n_iter < 0 || n_warp < 0 || (n_iter > 0 && (taus == nullptr || sigmas == nullptr || mus == nullptr)))
+        return VFS_EINVAL;
+    if (prm->median_k != 3 && prm->median_k != 5) return VFS_EINVAL;
+    if (n_warp > 0 && n_iter == 0) return VFS_EINVAL;  // the batched median reads this warp's thresholded u
+    const Variant* v = find_variant(tile_x, tile_y, k);
+    if (v == nullptr) return VFS_EINVAL;
+    hipStream_t s = S(stream);
+    const BatchLayout lay = batch_layout(n_pairs);
+    const unsigned char* base = static_cast<const unsigned char*>(d_workspace);
+    const PDLevel* pd = reinterpret_cast<const PDLevel*>(base + lay.pd);
+    const int* pd_start = reinterpret_cast<const int*>(base + lay.pd_start);
+    const WarpDesc* wd = reinterpret_cast<const WarpDesc*>(base + lay.wd);
+    const int* wd_start = reinterpret_cast<const int*>(base + lay.wd_start);
+    int pd_tiles = 0, px_tiles = 0;
+    for (int i = 0; i < n_pairs; ++i) {
+        VFS_TRY(check_pair(&lv[i], &ws[i]));
+        const vfs_level& l = lv[i];
+        pd_tiles += ((l.W + tile_x - 1) / tile_x) * ((l.H + tile_y - 1) / tile_y);
+        px_tiles += ((l.W + 63) / 64) * ((l.H + 3) / 4);
+        VFS_TRY(vfs_gaussian_masked(stream, l.ref_smoothed, l.ref, l.mask, l.H, l.W));
+        VFS_TRY(vfs_compute_tensor_eta(stream, l.abc, l.eta, l.ref_smoothed, l.mask, l.H, l.W, prm));
+        if (n_warp > 0)
+            hipLaunchKernelGGL(k_warp_prep, grid2d(l.H, l.W), block2d(), 0, s, reinterpret_cast<float2*>(l.tv),
+                               l.warped, reinterpret_cast<const float2*>(l.flow),
+                               reinterpret_cast<const float2*>(l.tvf), l.other, l.H, l.W);
+    }
+    const PDConsts cst = make_consts(prm);
+    int cur = 0;
+    for (int w = 0; w < n_warp; ++w) {
+        hipLaunchKernelGGL(k_derivatives_batched, dim3(px_tiles), dim3(256), 0, s, wd, wd_start, n_pairs);
+        int done = 0;
+        bool first = true;
+        while (done < n_iter) {
+            const int kc = (n_iter - done) < k ? (n_iter - done) : k;
+            const int table = (first ? 0 : 2) + cur;
+            const PDSched sc = make_sched(prm, taus + done, sigmas + done, mus + done, kc);
+            hipLaunchKernelGGL(v->batched, dim3(pd_tiles), dim3(v->nt), 0, s, pd + table * n_pairs, pd_start,
+                               n_pairs, kc, sc, cst);
+            cur = 1 - cur;
+            done += kc;
+            first = false;
+        }
+        const int prep = (w + 1 < n_warp) ? 1 : 0;
+        if (prm->median_k == 5)
+            hipLaunchKernelGGL(k_median_clamp_prep_batched<5>, dim3(px_tiles), dim3(256), 0, s, wd + cur * n_pairs,
+                               wd_start, n_pairs, prm->limit_range, prep);
+        else
+            hipLaunchKernelGGL(k_median_clamp_prep_batched<3>, dim3(px_tiles), dim3(256), 0, s, wd + cur * n_pairs,
+                               wd_start, n_pairs, prm->limit_range, prep);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return static_cast<int>(e);
+    }
+    if (cur == 1)
+        for (int i = 0; i < n_pairs; ++i)
+            VFS_TRY(vfs_copy(stream, lv[i].u_bar, ws[i].u_bar2, static_cast<int64_t>(lv[i].H) * lv[i].W));
     return 0;
 }
 

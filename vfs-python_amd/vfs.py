@@ -378,6 +378,38 @@ class VFS:
         self._bufs = b
         return b
 
+    def _enqueue_pyramids(self, s, b):
+        """castAndScale of both views, de-rotation of the other view, both pyramids (Python/vfs.py:121-157)."""
+        for d, src in ((b["ref0"], b["img_ref"]), (b["other0"], b["img_other"])):
+            H.call("vfs_cast_and_scale", s, H.ptr(d), H.ptr(src), d.shape[0], d.shape[1])
+        self._image_pyramid_into(s, b["refs"])
+        o0 = b["others"][0]
+        H.call("vfs_warp_image", s, H.ptr(o0), H.ptr(self.calibration), o0.shape[0], o0.shape[1], H.ptr(b["other0"]),
+               b["other0"].shape[0], b["other0"].shape[1])
+        self._image_pyramid_into(s, b["others"])
+
+    def _enqueue_level_init(self, s, b, k):
+        """State at the start of level k (1-based): zeros at the coarsest, else upsampled (Python/vfs.py:466-469,523-528)."""
+        p = self.param
+        sizes = self.sizes
+        L = b["levels"][k - 1]
+        hh, ww = sizes[k - 1]
+        if k == len(sizes):
+            for name in ("flow", "u", "u_bar"):
+                H.call("vfs_fill", s, H.ptr(L[name]), 0.0, L[name].numel())
+        else:
+            P = b["levels"][k]
+            ph, pw = sizes[k]
+            H.call("vfs_upsample", s, H.ptr(L["u_bar"]), hh, ww, H.ptr(P["u"]), ph, pw, 1, float(p.fScale))
+            H.call("vfs_copy", s, H.ptr(L["u"]), H.ptr(L["u_bar"]), L["u"].numel())
+            H.call("vfs_upsample", s, H.ptr(L["flow"]), hh, ww, H.ptr(P["flow"]), ph, pw, 2, float(p.fScale))
+
+    def _enqueue_result(self, s, b):
+        """flow[mask0 == 0] = 0 into the result buffer (Python/vfs.py:531-533)."""
+        L0 = b["levels"][0]
+        H.call("vfs_copy", s, H.ptr(b["result"]), H.ptr(L0["flow"]), L0["flow"].numel())
+        H.call("vfs_zero_masked", s, H.ptr(b["result"]), H.ptr(self.masks[0]), self.sizes[0][0], self.sizes[0][1], 2)
+
     def _enqueue_solve(self, s, b):
         """Enqueue the whole solve on stream ``s`` (recorded when ``s`` is a capture stream)."""
         p = self.param
@@ -386,27 +418,13 @@ class VFS:
         nL = len(sizes)
         blocked = self.solver in ("blocked", "blocked_eager")
         tiles = self._tiles() if blocked else None
-        for d, src in ((b["ref0"], b["img_ref"]), (b["other0"], b["img_other"])):
-            H.call("vfs_cast_and_scale", s, H.ptr(d), H.ptr(src), d.shape[0], d.shape[1])
-        self._image_pyramid_into(s, b["refs"])
-        o0 = b["others"][0]
-        H.call("vfs_warp_image", s, H.ptr(o0), H.ptr(self.calibration), o0.shape[0], o0.shape[1], H.ptr(b["other0"]),
-               b["other0"].shape[0], b["other0"].shape[1])
-        self._image_pyramid_into(s, b["others"])
+        self._enqueue_pyramids(s, b)
         taus, sigmas, mus = H.pd_schedule(int(p.nSolverIters), p.timeStepLambda)
         sched = [a.ctypes.data_as(ctypes.c_void_p) for a in (taus, sigmas, mus)]
         for k in range(nL, 0, -1):
             L = b["levels"][k - 1]
             hh, ww = sizes[k - 1]
-            if k == nL:
-                for name in ("flow", "u", "u_bar"):
-                    H.call("vfs_fill", s, H.ptr(L[name]), 0.0, L[name].numel())
-            else:
-                P = b["levels"][k]
-                ph, pw = sizes[k]
-                H.call("vfs_upsample", s, H.ptr(L["u_bar"]), hh, ww, H.ptr(P["u"]), ph, pw, 1, float(p.fScale))
-                H.call("vfs_copy", s, H.ptr(L["u"]), H.ptr(L["u_bar"]), L["u"].numel())
-                H.call("vfs_upsample", s, H.ptr(L["flow"]), hh, ww, H.ptr(P["flow"]), ph, pw, 2, float(p.fScale))
+            self._enqueue_level_init(s, b, k)
             lv = H.VfsLevel()
             lv.H, lv.W = hh, ww
             lv.ref = b["refs"][k - 1].data_ptr()
@@ -425,9 +443,7 @@ class VFS:
             else:
                 H.call("vfs_solve_level_plain", s, ctypes.byref(lv), ctypes.byref(prm), *sched,
                        int(p.nSolverIters), int(p.nWarpIters))
-        L0 = b["levels"][0]
-        H.call("vfs_copy", s, H.ptr(b["result"]), H.ptr(L0["flow"]), L0["flow"].numel())
-        H.call("vfs_zero_masked", s, H.ptr(b["result"]), H.ptr(self.masks[0]), sizes[0][0], sizes[0][1], 2)
+        self._enqueue_result(s, b)
 
     def _solve_stereo_python(self, img_ref, img_other):
         """Python/vfs.py:460-533 literally: one Python call per kernel."""

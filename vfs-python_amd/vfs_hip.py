@@ -85,6 +85,9 @@ _SIGS = {
     "vfs_pd_blocks": [c_p, c_i, c_i] + [c_p] * 14 + [c_i, c_p, c_p, c_p, c_i, c_i, c_i, c_i, ctypes.POINTER(VfsParams)],
     "vfs_solve_level_blocked": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsPdWs), ctypes.POINTER(VfsParams),
                                 c_p, c_p, c_p, c_i, c_i, c_i, c_i, c_i],
+    "vfs_batch_prepare_level": [c_p, ctypes.c_size_t, c_p, c_p, c_i, c_i, c_i, c_i],
+    "vfs_solve_level_blocked_batch": [c_p, c_p, c_p, c_p, c_i, ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i,
+                                      c_i, c_i, c_i],
     "vfs_capture_begin": [ctypes.POINTER(c_p)],
     "vfs_capture_end": [c_p, ctypes.POINTER(c_p)],
     "vfs_plan_launch": [c_p, c_p],
@@ -94,7 +97,7 @@ _SIGS = {
 
 def exported_symbols():
     """Names of the C ABI entry points this binding expects (include/vfs_hip.h)."""
-    return sorted(list(_SIGS) + ["vfs_version"])
+    return sorted(list(_SIGS) + ["vfs_version", "vfs_batch_workspace_bytes"])
 
 
 def lib():
@@ -110,6 +113,8 @@ def lib():
             fn.restype = c_i
         L.vfs_version.restype = ctypes.c_char_p
         L.vfs_version.argtypes = []
+        L.vfs_batch_workspace_bytes.restype = ctypes.c_size_t
+        L.vfs_batch_workspace_bytes.argtypes = [c_i]
         _lib = L
     return _lib
 
