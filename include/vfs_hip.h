@@ -290,6 +290,94 @@ int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const v
                                   const float* taus, const float* sigmas, const float* mus,
                                   int n_iter, int n_warp, int tile_x, int tile_y, int k);
 
+/* ---- adjoint (reverse mode) of the solve (no reference counterpart) -------------- */
+
+/* Buffers of one reverse primal-dual iteration at one pyramid level.  The forward
+ * iteration t maps (p, q, u, u_bar, v_bar)_{t-1} to (p, q, u, u_bar, v_bar)_t
+ * (Python/vfs.py:505-511); its reverse maps the adjoint of the state after t to
+ * the adjoint of the state before t.  Adjoint contributions addressed to a
+ * neighbouring pixel travel as per-pixel messages (um, vm, qmsg, tmsg) that the
+ * receiving pixel collects in the next reverse kernel; the adjoints of u_bar and
+ * v_bar therefore stay split into an own part (gub, gvb) and the messages until
+ * vfs_adj_pd_finish combines them. */
+typedef struct vfs_adj_pd {
+    int32_t H, W;
+    float* gu;      /* (H,W)   adjoint of u_t on entry, of u_{t-1} on exit          */
+    float* gub;     /* (H,W)   own part of the adjoint of u_bar_t / u_bar_{t-1}       */
+    float* gvb;     /* (H,W,2) own part of the adjoint of v_bar_t / v_bar_{t-1}       */
+    float* gp;      /* (H,W,2) adjoint of p_t on entry, of p_{t-1} on exit          */
+    float* gq;      /* (H,W,4) adjoint of q_t on entry, of q_{t-1} on exit          */
+    float* um;      /* (H,W,3) dual messages for u_bar (read on entry if has_msgs)   */
+    float* vm;      /* (H,W,6) dual messages for v_bar                              */
+    float* qmsg;    /* (H,W,4) scratch: primal messages for q                        */
+    float* tmsg;    /* (H,W,4) scratch: threshold messages for tp                    */
+    float* gabc;    /* (H,W,3) accumulated adjoint of abc                            */
+    float* geta;    /* (H,W,3) accumulated adjoint of eta                            */
+    float* gderiv;  /* (H,W,2) accumulated adjoint of deriv (Iu, Iz)                 */
+    float* gpar;    /* (H,W,5) accumulated lambda, alpha0, alpha1, beta, gamma       */
+    const float* p;          /* p_t   (H,W,2) */
+    const float* q;          /* q_t   (H,W,4) */
+    const float* u;          /* u_t   (H,W)   */
+    const float* p_prev;     /* p_{t-1} */
+    const float* q_prev;     /* q_{t-1} */
+    const float* u_prev;     /* u_{t-1} */
+    const float* u_bar_prev; /* u_bar_{t-1} */
+    const float* v_bar_prev; /* v_bar_{t-1} */
+    const float* abc;
+    const float* eta;
+    const float* deriv;
+    const uint8_t* mask;
+} vfs_adj_pd;
+
+/* Reverse of one primal-dual iteration with the schedule values of that iteration;
+ * has_msgs = 0 when no later iteration left messages in um / vm. */
+int vfs_adj_pd_iteration(void* stream, const vfs_adj_pd* a, const vfs_params* prm, float tau, float sigma,
+                         float mu, int has_msgs);
+
+/* Complete adjoints of u_bar and v_bar before the first iteration of a loop. */
+int vfs_adj_pd_finish(void* stream, float* gub_out, float* gvb_out, const float* gub, const float* gvb,
+                      const float* um, const float* vm, const uint8_t* mask, int H, int W);
+
+/* Reverse of computeOpticalFlowWithClamp after medianFilter (Python/vfs.cu:583-666):
+ * gf (H,W,2) is the adjoint of the flow after the warp (it is also the adjoint of
+ * the flow before); gu (H,W) the adjoint of u after the warp, replaced by the
+ * clamp's share of the adjoint of the warp-start u0.  Writes guf (adjoint of the
+ * median), pos (int32 window position of the median, -1 outside the mask), gtv
+ * (H,W,2, adjoint of the warping vector). */
+int vfs_adj_clamp(void* stream, float* gu, float* guf, int* pos, float* gtv, const float* gf,
+                  const float* u_th, const float* u0, const float* tv, const uint8_t* mask, int H, int W,
+                  float limit, int ksize);
+/* Reverse of medianFilter: gu_th (H,W) := adjoint of the thresholded u. */
+int vfs_adj_median(void* stream, float* gu_th, const float* guf, const int* pos, int H, int W, int ksize);
+/* Reverse of computeDerivatives (Python/vfs.cu:120-142): gref, gow (H,W) += (atomics). */
+int vfs_adj_derivatives(void* stream, float* gref, float* gow, const float* gderiv, const float* tvf,
+                        int H, int W);
+/* Reverse of warpingImage / findWarpingVector at the warp-start flow:
+ * gf (H,W,2) += gow dS(other)/dx + gtv . dS(tvf)/dx. */
+int vfs_adj_warp(void* stream, float* gf, const float* gow, const float* gtv, const float* flow,
+                 const float* other, const float* tvf, int H, int W);
+/* Reverse of computeTensorEtaMasked (Python/vfs.cu:228-276): gimg (H,W) += (atomics);
+ * gpar (H,W,5) += beta, gamma, alpha0, alpha1 shares. */
+int vfs_adj_tensor(void* stream, float* gimg, float* gpar, const float* gabc, const float* geta,
+                   const float* img, const uint8_t* mask, int H, int W, const vfs_params* prm);
+/* Reverse of gaussianMasked: gimg (H,W) += (atomics). */
+int vfs_adj_gaussian(void* stream, float* gimg, const float* gout, const uint8_t* mask, int H, int W);
+/* Reverse of downSamplingImage: gsrc (Hs,Ws) += (atomics). */
+int vfs_adj_downsample(void* stream, float* gsrc, int Hs, int Ws, const float* gout, int Hd, int Wd);
+/* Reverse of upSamplingVectorField(1): gsrc (Hs,Ws,channels) += f_scale S^T gout (atomics). */
+int vfs_adj_upsample(void* stream, float* gsrc, int Hs, int Ws, const float* gout, int Hd, int Wd,
+                     int channels, float f_scale);
+/* Depth loss |log z - log z_gt| per pixel from the flow (z of triangulateRays,
+ * Python/vfs.cu:707-735, camera (f, u0, v0), translation x / z components):
+ * term (H,W,2) = (loss, valid); gflow (H,W,2) = d loss / d flow (not divided by
+ * the number of valid pixels). */
+int vfs_depth_loss(void* stream, float* term, float* gflow, const float* flow, const float* z_gt,
+                   const uint8_t* mask, int H, int W, float f, float u0, float v0, float tx, float tz);
+/* out[c] = sum_i in[i * channels + c] in double, deterministic (wavefront, then
+ * block, then one final block); partial holds nparts * channels doubles. */
+int vfs_reduce_sum(void* stream, double* out, double* partial, int nparts, const float* in, int64_t n,
+                   int channels);
+
 /* Library version string (static storage). */
 const char* vfs_version(void);
 

@@ -1,0 +1,226 @@
+"""Hand-written adjoint (csrc/vfs_adjoint.hip, vfs_adjoint.py) against two references:
+autograd through the unrolled PyTorch float64 restatement (oracle/vfs_torch_ref.py),
+and central finite differences of the float64 NumPy oracle.
+
+Tolerances (fp32 state, fp32 adjoint accumulation, against float64 references):
+  * N reversed primal-dual iterations: every input adjoint within 1e-4 of the
+    autograd gradient relative to the largest entry (same fp32 inputs);
+  * full solve, parameters and reference image: cosine similarity >= 0.999 and
+    relative error <= 2e-2 per parameter against autograd through the float64 solve;
+  * finite differences (central, h = 1e-4 relative) of the float64 oracle loss:
+    per parameter within max(2e-2 relative, 3x the deviation of the same gradient
+    taken by autograd through the restatement run in float32).  The depth loss
+    is ill-conditioned in fp32 on small crops (the float32 restatement is 6% off
+    for beta on the 40 x 40 case), the same allowance the flow tolerances make.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import vfs_oracle as O
+from oracle import vfs_torch_ref as R
+
+pytestmark = pytest.mark.gpu
+
+THETA = dict(Lambda=50.0, alpha0=5.0, alpha1=1.0, beta=10.0, gamma=0.5)
+
+
+@pytest.fixture(scope="module")
+def T():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("n_iter", [1, 4])
+def test_pd_iterations_adjoint_matches_autograd(T, n_iter):
+    import vfs_hip as H
+    rng = np.random.default_rng(7 + n_iter)
+    Hh, Ww = 23, 31
+    f32 = np.float32
+    m = (rng.random((Hh, Ww)) > 0.15)
+    m[0, :] = False
+    st0 = dict(p=rng.normal(size=(Hh, Ww, 2)) * 0.3, q=rng.normal(size=(Hh, Ww, 4)) * 0.3, u=rng.normal(size=(Hh, Ww)),
+               ub=rng.normal(size=(Hh, Ww)), vb=rng.normal(size=(Hh, Ww, 2)) * 0.1)
+    abc = rng.random((Hh, Ww, 3))
+    eta = rng.random((Hh, Ww, 3)) * 20 + 1
+    deriv = rng.normal(size=(Hh, Ww, 2)) * 0.2
+    st0 = {k: v.astype(f32) for k, v in st0.items()}
+    abc, eta, deriv = abc.astype(f32), eta.astype(f32), deriv.astype(f32)
+    taus, sigmas, mus = H.pd_schedule(n_iter, 1.0)
+    wts = {k: rng.normal(size=v.shape) for k, v in st0.items()}
+    lam, a0, a1 = 50.0, 5.0, 1.0
+
+    # float64 autograd
+    tt = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in st0.items()}
+    tabc = torch.tensor(abc, dtype=torch.float64, requires_grad=True)
+    teta = torch.tensor(eta, dtype=torch.float64, requires_grad=True)
+    tder = torch.tensor(deriv, dtype=torch.float64, requires_grad=True)
+    tpar = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in (lam, a0, a1)]
+    tm = torch.tensor(m)
+    p, q, u, ub, vb = tt["p"], tt["q"], tt["u"], tt["ub"], tt["vb"]
+    for i in range(n_iter):
+        p, tp, q = R.update_dual(p, q, float(sigmas[i]), ub, vb, tabc, tm, tpar[1], tpar[2])
+        u = R.l1_thresholding(float(taus[i]), tp, u, ub, tder, teta, tm, tpar[0])
+        ub, vb = R.update_primal(float(taus[i]), float(mus[i]), p, q, u, ub, vb, tabc, teta, tm, tpar[1], tpar[2])
+    out = dict(p=p, q=q, u=u, ub=ub, vb=vb)
+    loss = sum((out[k] * torch.tensor(wts[k])).sum() for k in out)
+    loss.backward()
+
+    # HIP forward (tape) + reverse
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    s = H.stream()
+
+    class P:
+        beta, gamma, alpha0, alpha1, Lambda, limitRange, fScale = 10.0, 0.5, a0, a1, lam, 0.1, 2.0
+
+    prm = H.make_params(P)
+    mask = dev(m.astype(np.uint8) * 255)
+    cabc, ceta, cder = dev(abc), dev(eta), dev(deriv)
+    st = {k: dev(v) for k, v in st0.items()}
+    tape = [{k: v.clone() for k, v in st.items()}]
+    tpb = torch.empty_like(st["p"])
+    for i in range(n_iter):
+        H.call("vfs_update_dual", s, H.ptr(st["p"]), H.ptr(tpb), H.ptr(st["q"]), float(sigmas[i]), H.ptr(st["ub"]),
+               H.ptr(st["vb"]), H.ptr(cabc), H.ptr(mask), Hh, Ww, ctypes.byref(prm))
+        H.call("vfs_l1_thresholding", s, H.ptr(st["u"]), float(taus[i]), H.ptr(tpb), H.ptr(st["u"]), H.ptr(st["ub"]),
+               H.ptr(cder), H.ptr(ceta), H.ptr(mask), Hh, Ww, ctypes.byref(prm))
+        H.call("vfs_update_primal", s, H.ptr(st["ub"]), H.ptr(st["vb"]), float(taus[i]), float(mus[i]), H.ptr(st["p"]),
+               H.ptr(st["q"]), H.ptr(st["u"]), H.ptr(st["ub"]), H.ptr(st["vb"]), H.ptr(cabc), H.ptr(ceta), H.ptr(mask),
+               Hh, Ww, ctypes.byref(prm))
+        tape.append({k: v.clone() for k, v in st.items()})
+    z = lambda *c: torch.zeros((Hh, Ww, *c), device="cuda")  # noqa: E731
+    g = dict(gu=dev(wts["u"].astype(f32)), gub=dev(wts["ub"].astype(f32)), gvb=dev(wts["vb"].astype(f32)),
+             gp=dev(wts["p"].astype(f32)), gq=dev(wts["q"].astype(f32)), um=z(3), vm=z(6), qmsg=z(4), tmsg=z(4),
+             gabc=z(3), geta=z(3), gderiv=z(2), gpar=z(5))
+    has = 0
+    for i in range(n_iter - 1, -1, -1):
+        ad = H.VfsAdjPd()
+        ad.H, ad.W = Hh, Ww
+        for name, t in g.items():
+            setattr(ad, name, t.data_ptr())
+        after, before = tape[i + 1], tape[i]
+        for name, t in (("p", after["p"]), ("q", after["q"]), ("u", after["u"]), ("p_prev", before["p"]),
+                        ("q_prev", before["q"]), ("u_prev", before["u"]), ("u_bar_prev", before["ub"]),
+                        ("v_bar_prev", before["vb"]), ("abc", cabc), ("eta", ceta), ("deriv", cder), ("mask", mask)):
+            setattr(ad, name, t.data_ptr())
+        H.call("vfs_adj_pd_iteration", s, ctypes.byref(ad), ctypes.byref(prm), float(taus[i]), float(sigmas[i]),
+               float(mus[i]), has)
+        has = 1
+    gub_fin, gvb_fin = z(), z(2)
+    H.call("vfs_adj_pd_finish", s, H.ptr(gub_fin), H.ptr(gvb_fin), H.ptr(g["gub"]), H.ptr(g["gvb"]), H.ptr(g["um"]),
+           H.ptr(g["vm"]), H.ptr(mask), Hh, Ww)
+    torch.cuda.synchronize()
+    got = dict(p=g["gp"], q=g["gq"], u=g["gu"], ub=gub_fin, vb=gvb_fin)
+    for k in got:
+        r = _rel(got[k].cpu().numpy(), tt[k].grad.numpy())
+        print(k, r)
+        assert r <= 1e-4, k
+    assert _rel(g["gabc"].cpu().numpy(), tabc.grad.numpy()) <= 1e-4
+    assert _rel(g["geta"].cpu().numpy(), teta.grad.numpy()) <= 1e-4
+    assert _rel(g["gderiv"].cpu().numpy(), tder.grad.numpy()) <= 1e-4
+    par = g["gpar"].double().sum(dim=(0, 1)).cpu().numpy()
+    ref = [t.grad.item() for t in tpar]
+    for gi, ri in zip(par[:3], ref):
+        assert abs(gi - ri) <= 1e-4 * max(abs(ri), 1e-3) + 1e-3 * max(abs(x) for x in ref)
+
+
+def _small_case(n=48, seed=5, levels=2):
+    import synthetic as S
+    pr = S.make_pair(n, n, seed, t=(0.2, 0.0, 0.02), rot_deg=1.0, n_levels=levels)
+    cam = pr["cam"]
+    tvf, cvf = O.get_vector_fields(O.get_grid_zero((n, n)), cam, cam, pr["R"], pr["t"], 1)
+    return pr, S.sanitize_field(tvf), cvf.astype(np.float32)
+
+
+def _adjoint(pr, tvf, cvf, n_warp, n_iter, ckpt, levels=2):
+    import vfs_adjoint as A
+    import vfs_params as P
+    prm = P.vfs_params(pr["img_ref"])
+    prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = levels, 2.0, n_warp, n_iter
+    for k, v in THETA.items():
+        setattr(prm, k, v)
+    return A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=ckpt)
+
+
+def test_solve_adjoint_matches_autograd_through_unrolled_torch(T):
+    pr, tvf, cvf = _small_case()
+    n, n_warp, n_iter = 48, 2, 6
+    adj = _adjoint(pr, tvf, cvf, n_warp, n_iter, ckpt=4)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    loss = adj.forward(ref0, pr["img_other"])
+    g = adj.backward()
+    # float64 autograd through the same solve
+    sizes = O.compute_pyramid_sizes(n, n, 2, 2.0)
+    masks = [torch.as_tensor(m > 0) for m in O.create_mask_pyramid(pr["mask"], sizes)]
+    trs = [torch.as_tensor(t) for t in O.create_vector_field_pyramid(tvf, sizes)]
+    r0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"])).requires_grad_(True)
+    ow = torch.as_tensor(O.warping_image(O.cast_and_scale(pr["img_other"]), cvf))
+    th = [torch.tensor(THETA[k], dtype=torch.float64, requires_grad=True) for k in
+          ("Lambda", "alpha0", "alpha1", "beta", "gamma")]
+    flow = R.solve_stereo(r0, ow, masks, trs, sizes, *th, n_warp, n_iter)
+    lt = R.depth_loss(flow, pr["cam"], torch.as_tensor(pr["t"].reshape(3)), masks[0], torch.as_tensor(pr["z_gt"]))
+    lt.backward()
+    print("loss hip", loss, "torch", float(lt))
+    assert abs(loss - float(lt)) <= 1e-3 * abs(float(lt))
+    for i, k in enumerate(("Lambda", "alpha0", "alpha1", "beta", "gamma")):
+        ref = th[i].grad.item()
+        print(k, g[k], ref)
+        assert abs(g[k] - ref) <= 2e-2 * abs(ref) + 1e-6, k
+    gr, rr = g["ref0"].cpu().numpy().ravel(), r0.grad.numpy().ravel()
+    cos = float(np.dot(gr, rr) / (np.linalg.norm(gr) * np.linalg.norm(rr)))
+    print("ref0 cosine", cos, "rel", np.linalg.norm(gr - rr) / np.linalg.norm(rr))
+    assert cos >= 0.999
+
+
+def _oracle_loss(pr, tvf, cvf, theta, n_warp, n_iter, levels=2):
+    n = pr["img_ref"].shape[0]
+    prm = O.OracleParams(n, n, nLevel=levels, fScale=2.0, nWarpIters=n_warp, nSolverIters=n_iter, **theta)
+    flow = O.solve_stereo(pr["img_ref"], pr["img_other"], pr["mask"], tvf, cvf, prm)
+    z = O.triangulate_flow(flow, pr["cam"], pr["t"].reshape(3), pr["mask"])[..., 2]
+    m = (pr["mask"] > 0) & (z > 0) & np.isfinite(z)
+    return float(np.mean(np.abs(np.log(z[m]) - np.log(pr["z_gt"][m]))))
+
+
+def _torch_fp32_gradients(pr, tvf, cvf, n_warp, n_iter):
+    """Parameter gradients of the same loss by autograd through the restatement run in
+    float32 (on the CPU): the spread that fp32 arithmetic alone puts on the gradient."""
+    n = pr["img_ref"].shape[0]
+    f = torch.float32
+    sizes = O.compute_pyramid_sizes(n, n, 2, 2.0)
+    masks = [torch.as_tensor(m > 0) for m in O.create_mask_pyramid(pr["mask"], sizes)]
+    trs = [torch.as_tensor(t).to(f) for t in O.create_vector_field_pyramid(tvf, sizes)]
+    r0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"])).to(f)
+    ow = torch.as_tensor(O.warping_image(O.cast_and_scale(pr["img_other"]), cvf)).to(f)
+    th = [torch.tensor(THETA[k], dtype=f, requires_grad=True) for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma")]
+    flow = R.solve_stereo(r0, ow, masks, trs, sizes, *th, n_warp, n_iter)
+    R.depth_loss(flow, pr["cam"], torch.as_tensor(pr["t"].reshape(3)).to(f), masks[0],
+                 torch.as_tensor(pr["z_gt"]).to(f)).backward()
+    return [x.grad.item() for x in th]
+
+
+def test_solve_adjoint_matches_central_differences_of_the_oracle(T):
+    pr, tvf, cvf = _small_case(40, 9)
+    n_warp, n_iter = 2, 5
+    adj = _adjoint(pr, tvf, cvf, n_warp, n_iter, ckpt=2)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    adj.forward(ref0, pr["img_other"])
+    g = adj.backward()
+    g32 = _torch_fp32_gradients(pr, tvf, cvf, n_warp, n_iter)
+    for i, k in enumerate(("Lambda", "alpha0", "alpha1", "beta", "gamma")):
+        h = 1e-4 * THETA[k]
+        tp, tm = dict(THETA), dict(THETA)
+        tp[k] += h
+        tm[k] -= h
+        fd = (_oracle_loss(pr, tvf, cvf, tp, n_warp, n_iter) - _oracle_loss(pr, tvf, cvf, tm, n_warp, n_iter)) / (2 * h)
+        tol = max(2e-2 * abs(fd), 3.0 * abs(g32[i] - fd)) + 1e-5
+        print(k, "adjoint", g[k], "central difference", fd, "fp32 restatement", g32[i])
+        assert abs(g[k] - fd) <= tol, k

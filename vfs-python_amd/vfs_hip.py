@@ -50,6 +50,16 @@ class VfsPdWs(ctypes.Structure):
     _fields_ = [(n, c_p) for n in ("u_th0", "u_th1", "u_bar2", "v_bar2", "p2", "q2")]
 
 
+class VfsAdjPd(ctypes.Structure):
+    """struct vfs_adj_pd (include/vfs_hip.h)."""
+
+    _fields_ = [("H", ctypes.c_int32), ("W", ctypes.c_int32)] + [
+        (n, c_p) for n in ("gu", "gub", "gvb", "gp", "gq", "um", "vm", "qmsg", "tmsg", "gabc", "geta", "gderiv",
+                           "gpar", "p", "q", "u", "p_prev", "q_prev", "u_prev", "u_bar_prev", "v_bar_prev", "abc",
+                           "eta", "deriv", "mask")
+    ]
+
+
 # name -> argtypes (restype is int for all but vfs_version)
 _SIGS = {
     "vfs_cast_and_scale": [c_p, c_p, c_p, c_i, c_i],
@@ -88,6 +98,18 @@ _SIGS = {
     "vfs_batch_prepare_level": [c_p, ctypes.c_size_t, c_p, c_p, c_i, c_i, c_i, c_i],
     "vfs_solve_level_blocked_batch": [c_p, c_p, c_p, c_p, c_i, ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i,
                                       c_i, c_i, c_i],
+    "vfs_adj_pd_iteration": [c_p, ctypes.POINTER(VfsAdjPd), ctypes.POINTER(VfsParams), c_f, c_f, c_f, c_i],
+    "vfs_adj_pd_finish": [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i, c_i],
+    "vfs_adj_clamp": [c_p] * 10 + [c_i, c_i, c_f, c_i],
+    "vfs_adj_median": [c_p, c_p, c_p, c_p, c_i, c_i, c_i],
+    "vfs_adj_derivatives": [c_p, c_p, c_p, c_p, c_p, c_i, c_i],
+    "vfs_adj_warp": [c_p] * 7 + [c_i, c_i],
+    "vfs_adj_tensor": [c_p] * 7 + [c_i, c_i, ctypes.POINTER(VfsParams)],
+    "vfs_adj_gaussian": [c_p, c_p, c_p, c_p, c_i, c_i],
+    "vfs_adj_downsample": [c_p, c_p, c_i, c_i, c_p, c_i, c_i],
+    "vfs_adj_upsample": [c_p, c_p, c_i, c_i, c_p, c_i, c_i, c_i, c_f],
+    "vfs_depth_loss": [c_p] * 6 + [c_i, c_i, c_f, c_f, c_f, c_f, c_f],
+    "vfs_reduce_sum": [c_p, c_p, c_p, c_i, c_p, c_i64, c_i],
     "vfs_capture_begin": [ctypes.POINTER(c_p)],
     "vfs_capture_end": [c_p, ctypes.POINTER(c_p)],
     "vfs_plan_launch": [c_p, c_p],
