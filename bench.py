@@ -40,7 +40,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup_solver(name, seed, solver):
+def setup_solver(name, seed, solver, device=None):
     import torch
     import vfs as V
     import vfs_params as P
@@ -48,7 +48,7 @@ def setup_solver(name, seed, solver):
     import equidistant_cam as E
     import synthetic as S
 
-    pair = S.workload_pair(name, seed=seed)
+    pair = S.workload_pair(name, seed=seed, device=device)
     cam = E.EquidistantCam(pair["cam"])
     cam.setup_module()
     Hh, Ww = pair["img_ref"].shape
@@ -242,6 +242,96 @@ def run_batch(a):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(a):
+    """Gradient of the depth loss (lambda, alpha0, alpha1, beta, gamma, reference image) for
+    `pairs` seeded 800x800 pairs, 3 levels x 10 warps x 50 iterations, checkpoint every 25.
+    A step is forward + backward of every pair.  The plain (per-kernel, graph) and blocked
+    forward solves of the same schedule are timed beside it for the cost ratio."""
+    import torch
+    import synthetic as S
+    import vfs_adjoint as A
+
+    name = "adjoint_800"
+    w = S.WORKLOADS[name]
+    n_pairs = min(a.pairs, w["pairs"]) if a.pairs else w["pairs"]
+    log(f"[bench] workload {name}: {n_pairs} pairs")
+    ctxs, adjs = [], []
+    for i in range(n_pairs):
+        ctx = setup_solver(name, a.seed + i, "graph", device="cuda")
+        pr = ctx["pair"]
+        adjs.append(A.VFSDepthAdjoint(ctx["prm"], pr["mask"], ctx["tvf"], ctx["cvf"], pr["cam"], pr["t"],
+                                      pr["z_gt"], checkpoint=w["checkpoint"]))
+        ctxs.append(ctx)
+    refs0 = [c["img_ref"].to(torch.float32) / 256.0 for c in ctxs]
+
+    def clock(fn, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    v = ctxs[0]["vfs"]
+    res = {}
+    for solver in ("graph", "blocked"):
+        v.solver = solver
+        for _ in range(a.warmup):
+            v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"])
+        res[solver] = clock(lambda: v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"]), max(a.steps, 3))
+        log(f"[bench] forward solve ({solver}): {1e3 * res[solver]:.2f} ms")
+    adj = adjs[0]
+    for _ in range(a.warmup):
+        adj.forward(refs0[0], ctxs[0]["img_other"])
+        adj.backward()
+    t_fwd = clock(lambda: adj.forward(refs0[0], ctxs[0]["img_other"]), max(a.steps, 3))
+    t_bwd = clock(adj.backward, max(a.steps, 3))
+    log(f"[bench] adjoint forward (with saves) {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms")
+
+    losses, grads = [], []
+
+    def step():
+        losses.clear()
+        grads.clear()
+        for i in range(n_pairs):
+            losses.append(adjs[i].forward(refs0[i], ctxs[i]["img_other"]))
+            g = adjs[i].backward()
+            grads.append([g[k] for k in A.PARAMS])
+
+    log(f"[bench] timing {a.steps} steps of {n_pairs} pair gradients")
+    for _ in range(a.warmup):
+        step()
+    dt = clock(step, a.steps)
+    gmean = np.mean(np.asarray(grads), axis=0)
+    out = {
+        "metric": METRIC,
+        "value": round(n_pairs / dt, 3),
+        "unit": "pair gradients/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * dt, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": name, "size": [w["H"], w["W"]], "levels": w["nLevel"],
+                   "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
+                   "checkpoint": w["checkpoint"], "pairs": n_pairs},
+        "forward_solve_ms": {k: round(1e3 * t, 3) for k, t in res.items()},
+        "adjoint_forward_ms": round(1e3 * t_fwd, 3),
+        "adjoint_backward_ms": round(1e3 * t_bwd, 3),
+        "backward_vs_plain_forward": round(t_bwd / res["graph"], 3),
+        "gradient_vs_plain_forward": round((t_fwd + t_bwd) / res["graph"], 3),
+        "mean_loss": round(float(np.mean(losses)), 6),
+        "mean_gradient": {k: float(x) for k, x in zip(A.PARAMS, gmean)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_oracle_baseline(ctx, name):
     """float64 NumPy oracle (1 thread) on the full pyramid with 1 of nWarpIters warps per level."""
     from oracle import vfs_oracle as O
@@ -281,7 +371,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain_full_800", choices=["plain_full_800", "plain_small_256", "batched_64"])
+    ap.add_argument("--workload", default="plain_full_800",
+                    choices=["plain_full_800", "plain_small_256", "batched_64", "adjoint_800"])
     ap.add_argument("--pairs", type=int, default=64, help="batched_64: number of pairs (first n of the 64)")
     ap.add_argument("--solver", default="blocked", choices=["graph", "eager", "blocked", "blocked_eager"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain per-kernel path beside it")
@@ -300,6 +391,9 @@ def main():
     torch.cuda.set_device(0)
     if a.workload == "batched_64":
         run_batch(a)
+        return
+    if a.workload == "adjoint_800":
+        run_adjoint(a)
         return
     w = S.WORKLOADS[a.workload]
     log(f"[bench] workload {a.workload}: generating the pair and vector fields")

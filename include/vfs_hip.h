@@ -378,6 +378,41 @@ int vfs_depth_loss(void* stream, float* term, float* gflow, const float* flow, c
 int vfs_reduce_sum(void* stream, double* out, double* partial, int nparts, const float* in, int64_t n,
                    int channels);
 
+/* ---- adjoint level driver ---------------------------------------------------------
+ * The forward warp loop of one level that records what the reverse needs, and the
+ * reverse sweep of one level, each enqueued on one stream without host
+ * synchronisation or memsets (both can be recorded into a HIP graph).
+ *
+ * Save area of one level (vfs_adj_save_bytes): per warp the warp-start u, u_bar and
+ * flow and ceil(n_iter / ckpt) checkpoints of (p, q, u, u_bar, v_bar), taken before
+ * iterations 0, ckpt, 2 ckpt, ...  Workspace (vfs_adj_workspace_bytes, one for the
+ * largest level serves every level): a tape of ckpt states that a checkpoint
+ * segment is recomputed into, plus the adjoint buffers.  Both must be 256-byte
+ * aligned (torch allocations are). */
+size_t vfs_adj_save_bytes(int H, int W, int n_warp, int n_iter, int ckpt);
+size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
+
+/* Same results as vfs_solve_level_plain (the per-kernel warp loop of
+ * Python/vfs.py:473-521), plus the save area. */
+int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
+                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save);
+
+/* Reverse of vfs_adj_forward_level.  lv must still hold the forward's ref_smoothed,
+ * abc and eta; its tv, warped, deriv, p, q, tp, v_bar, u0, uf are overwritten.
+ * In: gu, gub (H,W) and gf (H,W,2) are the adjoints of the level's final u, u_bar
+ * and flow; on return they are the adjoints of the level's initial u, u_bar and
+ * flow.  gref (H,W) += adjoint of the level's reference image; gpar (H,W,5) +=
+ * shares of lambda, alpha0, alpha1, beta, gamma whose sum over all pixels is the
+ * level's contribution to those gradients.
+ * mode 0: the primal-dual reverse runs as one fused kernel per iteration
+ * (dual^T of iteration i with primal^T and threshold^T of iteration i - 1, the
+ * dual messages of a 32 x 8 tile and its one-pixel ring exchanged in LDS);
+ * mode 1: one kernel per step (vfs_adj_pd_iteration), messages through memory. */
+int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
+                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt,
+                          const float* save, float* workspace, float* gu, float* gub, float* gf, float* gref,
+                          float* gpar, int mode);
+
 /* Library version string (static storage). */
 const char* vfs_version(void);
 

@@ -224,3 +224,75 @@ def test_solve_adjoint_matches_central_differences_of_the_oracle(T):
         tol = max(2e-2 * abs(fd), 3.0 * abs(g32[i] - fd)) + 1e-5
         print(k, "adjoint", g[k], "central difference", fd, "fp32 restatement", g32[i])
         assert abs(g[k] - fd) <= tol, k
+
+
+def test_forward_level_driver_matches_plain_solver(T):
+    """vfs_adj_forward_level (plain warp loop + saves) gives the plain solver's flow bit for bit."""
+    import vfs as V
+    import vfs_params as P
+    pr, tvf, cvf = _small_case(64, 3)
+    adj = _adjoint(pr, tvf, cvf, 3, 7, ckpt=3)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    adj.forward(ref0, pr["img_other"])
+    prm = P.vfs_params(pr["img_ref"])
+    prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = 2, 2.0, 3, 7
+    for k, v in THETA.items():
+        setattr(prm, k, v)
+    v = V.VFS(prm)
+    v.solver = "eager"
+    v.setup_module()
+    v.compute_pyramid_sizes()
+    v.create_mask_pyramid(pr["mask"])
+    v.create_vector_field_pyramid(tvf, cvf)
+    flow = v.solve_stereo(pr["img_ref"], pr["img_other"])
+    torch.cuda.synchronize()
+    assert torch.equal(flow, adj.flow)
+
+
+def test_gradients_do_not_depend_on_the_checkpoint_interval(T):
+    pr, tvf, cvf = _small_case(48, 4)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    res = []
+    for ck in (1, 4, 9):
+        adj = _adjoint(pr, tvf, cvf, 2, 9, ckpt=ck)
+        loss = adj.forward(ref0, pr["img_other"])
+        res.append((loss, adj.backward()))
+    l0, g0 = res[0]
+    for loss, g in res[1:]:
+        assert loss == l0
+        for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma"):
+            # the bilinear transposes add with float atomics: their order may differ
+            assert abs(g[k] - g0[k]) <= 1e-5 * abs(g0[k]) + 1e-9, k
+        assert float((g["ref0"] - g0["ref0"]).abs().max()) <= 1e-5 * float(g0["ref0"].abs().max())
+
+
+@pytest.mark.parametrize("shape,ckpt", [((54, 70), 4), ((48, 48), 9)])
+def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt):
+    """The fused reverse sweep (dual^T + primal^T + threshold^T per launch, ring messages in
+    LDS) against the one-kernel-per-step sweep; partial tiles at the right / bottom edges."""
+    import synthetic as S
+    import vfs_adjoint as A
+    import vfs_params as P
+    hh, ww = shape
+    pr = S.make_pair(hh, ww, 8, t=(0.2, 0.0, 0.02), rot_deg=1.0, n_levels=2)
+    tvf, cvf = O.get_vector_fields(O.get_grid_zero((hh, ww)), pr["cam"], pr["cam"], pr["R"], pr["t"], 1)
+    tvf, cvf = S.sanitize_field(tvf), cvf.astype(np.float32)
+    prm = P.vfs_params(pr["img_ref"])
+    prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = 2, 2.0, 3, 9
+    for k, v in THETA.items():
+        setattr(prm, k, v)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    res = []
+    for fused in (False, True):
+        adj = A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=ckpt,
+                                fused=fused)
+        loss = adj.forward(ref0, pr["img_other"])
+        res.append((loss, adj.backward()))
+    (l0, g0), (l1, g1) = res
+    assert l0 == l1
+    for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma"):
+        print(k, g0[k], g1[k])
+        assert abs(g1[k] - g0[k]) <= 1e-5 * abs(g0[k]) + 1e-9, k
+    d = float((g1["ref0"] - g0["ref0"]).abs().max())
+    print("ref0 max diff", d, "max", float(g0["ref0"].abs().max()))
+    assert d <= 1e-5 * float(g0["ref0"].abs().max())

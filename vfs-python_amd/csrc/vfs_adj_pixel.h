@@ -1,0 +1,368 @@
+// Per-pixel arithmetic of the reverse primal-dual iteration, shared by the
+// one-kernel-per-step path (vfs_adjoint.hip) and the fused level sweep
+// (vfs_adjoint_solver.hip), so both compute the same values.
+//
+// The forward iteration t (Python/vfs.py:505-511) is dual (Python/vfs.cu:326-419),
+// threshold (:421-505), primal (:507-581).  Its reverse runs primal^T,
+// threshold^T, dual^T.  A contribution to a neighbour's adjoint is written as a
+// per-pixel message and collected by the receiving pixel, never scattered.
+//
+// Non-smooth points: projections are active when the norm exceeds 1; the
+// threshold uses the branch its forward took.
+#pragma once
+
+#include "vfs_device.h"
+#include "vfs_pd.h"
+
+namespace vfs {
+
+enum { GP_LAMBDA = 0, GP_ALPHA0 = 1, GP_ALPHA1 = 2, GP_BETA = 3, GP_GAMMA = 4, GP_N = 5 };
+
+struct Nbr {
+    int iE, iW, iS, iN;
+    bool m, mE, mW, mS, mN;
+};
+
+__device__ __forceinline__ Nbr nbr(const uint8_t* __restrict__ mask, int X, int Y, int H, int W)
+{
+    Nbr n;
+    n.iE = Y * W + clamp_idx(X + 1, W);
+    n.iW = Y * W + clamp_idx(X - 1, W);
+    n.iS = clamp_idx(Y + 1, H) * W + X;
+    n.iN = clamp_idx(Y - 1, H) * W + X;
+    n.m = mask[Y * W + X] != 0;
+    n.mE = mask[n.iE] != 0;
+    n.mW = mask[n.iW] != 0;
+    n.mS = mask[n.iS] != 0;
+    n.mN = mask[n.iN] != 0;
+    return n;
+}
+
+// x / y neighbour whose u_bar, v_bar the dual step of pixel (X, Y) reads
+__device__ __forceinline__ int dual_x_target(const uint8_t* __restrict__ mask, int X, int Y, int W)
+{
+    const int iE = Y * W + clamp_idx(X + 1, W);
+    return mask[iE] != 0 ? iE : Y * W + clamp_idx(X - 1, W);
+}
+
+__device__ __forceinline__ int dual_y_target(const uint8_t* __restrict__ mask, int X, int Y, int H, int W)
+{
+    const int iS = clamp_idx(Y + 1, H) * W + X;
+    return mask[iS] != 0 ? iS : clamp_idx(Y - 1, H) * W + X;
+}
+
+// Dual^T messages of one pixel: um = (own, to x-target, to y-target) for u_bar,
+// vm = (own.x, own.y, x-target.x, x-target.y, y-target.x, y-target.y) for v_bar.
+struct DualMsg {
+    float um[3];
+    float vm[6];
+};
+
+// Collect the messages addressed to pixel (X, Y), in the order: own, the x
+// senders X-1, X, X+1, the y senders Y-1, Y, Y+1.  msg(i) returns the DualMsg of
+// pixel index i (from global memory or from a tile in LDS).
+template <typename MsgAt>
+__device__ __forceinline__ void gather_msgs(const uint8_t* __restrict__ mask, MsgAt msg, int X, int Y, int H, int W,
+                                            float& gub, float2& gvb)
+{
+    const int j = Y * W + X;
+    {
+        const DualMsg m = msg(X, Y);
+        gub += m.um[0];
+        gvb.x += m.vm[0];
+        gvb.y += m.vm[1];
+    }
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int xs = X + dx;
+        if (xs < 0 || xs >= W) continue;
+        const int i = Y * W + xs;
+        if (mask[i] == 0 || dual_x_target(mask, xs, Y, W) != j) continue;
+        const DualMsg m = msg(xs, Y);
+        gub += m.um[1];
+        gvb.x += m.vm[2];
+        gvb.y += m.vm[3];
+    }
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int ys = Y + dy;
+        if (ys < 0 || ys >= H) continue;
+        const int i = ys * W + X;
+        if (mask[i] == 0 || dual_y_target(mask, X, ys, H, W) != j) continue;
+        const DualMsg m = msg(X, ys);
+        gub += m.um[2];
+        gvb.x += m.vm[4];
+        gvb.y += m.vm[5];
+    }
+}
+
+// Reverse of updatePrimalVariables (Python/vfs.cu:507-570) at a masked-in pixel.
+// g_ubo / g_vbo: complete adjoints of u_bar_t / v_bar_t.  State: q_t at the pixel
+// and its W / N neighbours (zero where masked), p_t, abc, eta.y, eta.z.
+// Sets gubB / gvbB (own part of the adjoints of u_bar_{t-1} / v_bar_{t-1}), qmsg;
+// adds to gu (adjoint of u_t), gp, gq, gabc, geta.y/.z, and the alpha shares.
+__device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4 qW, float4 qN, float a, float b,
+                                                 float c, float2 pc, float e1, float e2, float tau, float mu,
+                                                 float alpha0, float alpha1, float g_ubo, float2 g_vbo,
+                                                 float& gubB, float2& gvbB, float& gu, float2& gp, float4& gq,
+                                                 float4& qmsg, float (&gabc)[3], float& geta1, float& geta2,
+                                                 float& gpa0, float& gpa1)
+{
+    float4 gQ = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (n.mE) {
+        gQ.x = qc.x;
+        gQ.w = qc.w;
+    }
+    if (n.mW) {
+        gQ.x -= qW.x;
+        gQ.w -= qW.w;
+    }
+    if (n.mS) {
+        gQ.y = qc.y;
+        gQ.z = qc.z;
+    }
+    if (n.mN) {
+        gQ.y -= qN.y;
+        gQ.z -= qN.z;
+    }
+    const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
+    const bool d1 = !(fabsf(e1) < kEps), d2 = !(fabsf(e2) < kEps);
+    const float t1 = d1 ? tau / e1 : tau;
+    const float t2 = d2 ? tau / e2 : tau;
+    const float S1 = gQ.x + gQ.z, S2 = gQ.w + gQ.y;
+    const float A1 = fmaf(alpha1, tu.x, alpha0 * S1), A2 = fmaf(alpha1, tu.y, alpha0 * S2);
+    // ub_out = (1 + mu) u - mu ub ; vb_out = (1 + mu) v - mu vb ; v = t A + vb
+    const float g_u = (1.0f + mu) * g_ubo;
+    const float g_vx = (1.0f + mu) * g_vbo.x, g_vy = (1.0f + mu) * g_vbo.y;
+    gubB = -mu * g_ubo;
+    gvbB = make_float2(g_vx - mu * g_vbo.x, g_vy - mu * g_vbo.y);
+    gu += g_u;
+    const float g_A1 = g_vx * t1, g_A2 = g_vy * t2;
+    const float g_t1 = g_vx * A1, g_t2 = g_vy * A2;
+    gpa1 += g_A1 * tu.x + g_A2 * tu.y;
+    gpa0 += g_A1 * S1 + g_A2 * S2;
+    if (d1) geta1 += -g_t1 * tau / (e1 * e1);
+    if (d2) geta2 += -g_t2 * tau / (e2 * e2);
+    const float2 g_tu = make_float2(g_A1 * alpha1, g_A2 * alpha1);
+    gp.x += a * g_tu.x + c * g_tu.y;
+    gp.y += c * g_tu.x + b * g_tu.y;
+    gabc[0] += g_tu.x * pc.x;
+    gabc[1] += g_tu.y * pc.y;
+    gabc[2] += g_tu.x * pc.y + g_tu.y * pc.x;
+    const float g_S1 = g_A1 * alpha0, g_S2 = g_A2 * alpha0;
+    if (n.mE) {
+        gq.x += g_S1;
+        gq.w += g_S2;
+    }
+    if (n.mS) {
+        gq.z += g_S1;
+        gq.y += g_S2;
+    }
+    // messages: to W (x, w components of q_W), to N (y, z components of q_N)
+    qmsg = make_float4(n.mW ? -g_S1 : 0.0f, n.mW ? -g_S2 : 0.0f, n.mN ? -g_S2 : 0.0f, n.mN ? -g_S1 : 0.0f);
+}
+
+// T p of a pixel (zero where masked): the tp the forward threshold read.
+__device__ __forceinline__ float2 tp_at(const float2* __restrict__ p, const float* __restrict__ abc,
+                                        const uint8_t* __restrict__ mask, int i)
+{
+    if (mask[i] == 0) return make_float2(0.0f, 0.0f);
+    const float2 pp = p[i];
+    const float a = abc[3 * i], b = abc[3 * i + 1], c = abc[3 * i + 2];
+    return make_float2(fmaf(a, pp.x, c * pp.y), fmaf(c, pp.x, b * pp.y));
+}
+
+// Reverse of l1Thresholding (Python/vfs.cu:421-495) at a masked-in pixel.
+// gout: complete adjoint of u_t.  tc, tpWx, tpNy: T p_t at the pixel, x of its W
+// neighbour, y of its N neighbour.  State: u_{t-1}, u_bar_{t-1}, deriv, eta.x.
+// Returns the adjoint of u_{t-1}; sets tmsg = (own tp.x, own tp.y, to W (x), to N (y));
+// adds to gubB, geta.x, gderiv and the lambda share.
+__device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 tc, float tpWx, float tpNy, float2 d,
+                                              float e0, float uc, float ubc, float tau, float lambda, float4& tmsg,
+                                              float& gubB, float& geta0, float2& gderiv, float& gplam)
+{
+    float cxc, cxw, cyc, cyn;
+    if (n.mE && n.mW) {
+        cxc = 1.0f;
+        cxw = -1.0f;
+    } else if (!n.mE) {
+        cxc = 0.0f;
+        cxw = -1.0f;
+    } else {
+        cxc = 1.0f;
+        cxw = 0.0f;
+    }
+    if (n.mS && n.mN) {
+        cyc = 1.0f;
+        cyn = -1.0f;
+    } else if (!n.mS) {
+        cyc = 0.0f;
+        cyn = -1.0f;
+    } else {
+        cyc = 1.0f;
+        cyn = 0.0f;
+    }
+    const float gx = (n.mE && n.mW) ? tc.x - tpWx : (!n.mE ? -tpWx : tc.x);
+    const float gy = (n.mS && n.mN) ? tc.y - tpNy : (!n.mS ? -tpNy : tc.y);
+    const float dIu = d.x, dIz = d.y;
+    const bool de = !(fabsf(e0) < kEps);
+    const float te = de ? tau / e0 : tau;
+    const float ut = ubc + te * (gx + gy);
+    const float rho = dIu * (ut - uc) + dIz;
+    const float bound = lambda * te * dIu * dIu;
+    const float d0 = ut - uc;
+    float g_d0 = 0.0f, g_dIu = 0.0f, g_dIz = 0.0f, g_te = 0.0f, g_lam = 0.0f;
+    if ((rho <= bound) && (rho >= -bound)) {
+        if (fabsf(dIu) < kEps) {
+            g_d0 = gout;
+        } else {  // du = d0 - rho / dIu
+            const float g_rho = -gout / dIu;
+            g_d0 = gout + g_rho * dIu;
+            g_dIu = gout * rho / (dIu * dIu) + g_rho * d0;
+            g_dIz = g_rho;
+        }
+    } else if (rho < -bound) {  // du = d0 + lambda te dIu
+        g_d0 = gout;
+        g_lam = gout * te * dIu;
+        g_te = gout * lambda * dIu;
+        g_dIu = gout * lambda * te;
+    } else if (rho > bound) {  // du = d0 - lambda te dIu
+        g_d0 = gout;
+        g_lam = -gout * te * dIu;
+        g_te = -gout * lambda * dIu;
+        g_dIu = -gout * lambda * te;
+    }
+    // out = uc + du, d0 = ut - uc, ut = ubc + te (gx + gy)
+    const float g_ut = g_d0;
+    gubB += g_ut;
+    g_te += g_ut * (gx + gy);
+    const float g_g = g_ut * te;
+    if (de) geta0 += -g_te * tau / (e0 * e0);
+    gderiv.x += g_dIu;
+    gderiv.y += g_dIz;
+    gplam += g_lam;
+    tmsg = make_float4(cxc * g_g, cyc * g_g, cxw * g_g, cyn * g_g);
+    return gout - g_d0;
+}
+
+// Adjoints of q_t and tp_t at pixel idx completed with the messages of the
+// primal^T and threshold^T steps (senders: the pixel itself at the image's W / N
+// edge, its E and S neighbours).
+__device__ __forceinline__ void collect_qt(const float4* __restrict__ qmsg, const float4* __restrict__ tmsg,
+                                           const Nbr& n, int X, int Y, int H, int W, int idx, float4& gqn,
+                                           float2& gtp)
+{
+    const float4 m0 = qmsg[idx];
+    if (n.iW == idx) {
+        gqn.x += m0.x;
+        gqn.w += m0.y;
+    }
+    if (X + 1 < W) {
+        const float4 me = qmsg[idx + 1];
+        gqn.x += me.x;
+        gqn.w += me.y;
+    }
+    if (n.iN == idx) {
+        gqn.y += m0.z;
+        gqn.z += m0.w;
+    }
+    if (Y + 1 < H) {
+        const float4 ms = qmsg[idx + W];
+        gqn.y += ms.z;
+        gqn.z += ms.w;
+    }
+    const float4 t0 = tmsg[idx];
+    gtp = make_float2(t0.x, t0.y);
+    if (n.iW == idx) gtp.x += t0.z;
+    if (X + 1 < W) gtp.x += tmsg[idx + 1].z;
+    if (n.iN == idx) gtp.y += t0.w;
+    if (Y + 1 < H) gtp.y += tmsg[idx + W].w;
+}
+
+// Reverse of updateDualVariables (Python/vfs.cu:326-409) at a masked-in pixel.
+// gpn, gqn, gtp: complete adjoints of p_t, q_t, tp_t.  State: p_{t-1}, q_{t-1} at the
+// pixel; u_bar_{t-1}, v_bar_{t-1} at the pixel and its x / y targets ix, iy (E / S
+// when present, else W / N; sx, sy = +1 / -1).
+// Sets gp, gq := adjoints of p_{t-1}, q_{t-1} and the messages; adds to gabc and
+// the alpha shares.
+__device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float2 vc, float uX, float2 vX, float uY,
+                                               float2 vY, float a, float b, float c, float2 pc, float4 qc, float sigma,
+                                               float alpha0, float alpha1, float eta_p, float eta_q, float2 gpn,
+                                               float4 gqn, float2 gtp, float2& gp_out, float4& gq_out, DualMsg& msg,
+                                               float (&gabc)[3], float& gpa0, float& gpa1)
+{
+    const float sx = fx ? 1.0f : -1.0f, sy = fy ? 1.0f : -1.0f;
+    float2 gU;
+    float4 gV;
+    if (fx) {
+        gU.x = uX - uc - vc.x;
+        gV.x = vX.x - vc.x;
+        gV.w = vX.y - vc.y;
+    } else {
+        gU.x = uc - uX - vc.x;
+        gV.x = vc.x - vX.x;
+        gV.w = vc.y - vX.y;
+    }
+    if (fy) {
+        gU.y = uY - uc - vc.y;
+        gV.y = vY.y - vc.y;
+        gV.z = vY.x - vc.x;
+    } else {
+        gU.y = uc - uY - vc.y;
+        gV.y = vc.y - vY.y;
+        gV.z = vc.x - vY.x;
+    }
+    const float sp = dual_scale(alpha1, sigma, eta_p), sq = dual_scale(alpha0, sigma, eta_q);
+    const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
+    const float2 up = make_float2(fmaf(sp, tu.x, pc.x), fmaf(sp, tu.y, pc.y));
+    const float np_ = hypotf(up.x, up.y);
+    const bool projp = fmaxf(np_, 1.0f) > 1.0f;
+    const float2 pn = projp ? make_float2(up.x / np_, up.y / np_) : up;
+    const float4 uq = make_float4(fmaf(sq, gV.x, qc.x), fmaf(sq, gV.y, qc.y), fmaf(sq, gV.z, qc.z),
+                                  fmaf(sq, gV.w, qc.w));
+    const float nq = norm4df(uq.x, uq.y, uq.z, uq.w);
+    const bool projq = fmaxf(nq, 1.0f) > 1.0f;
+    const float4 qn = projq ? make_float4(uq.x / nq, uq.y / nq, uq.z / nq, uq.w / nq) : uq;
+    // tp = T pn
+    float ga = gtp.x * pn.x, gb = gtp.y * pn.y, gc = gtp.x * pn.y + gtp.y * pn.x;
+    gpn.x += a * gtp.x + c * gtp.y;
+    gpn.y += c * gtp.x + b * gtp.y;
+    // projection of p
+    float2 gup = gpn;
+    if (projp) {
+        const float dp = pn.x * gpn.x + pn.y * gpn.y;
+        gup = make_float2((gpn.x - pn.x * dp) / np_, (gpn.y - pn.y * dp) / np_);
+    }
+    float4 guq = gqn;
+    if (projq) {
+        const float dq = qn.x * gqn.x + qn.y * gqn.y + qn.z * gqn.z + qn.w * gqn.w;
+        guq = make_float4((gqn.x - qn.x * dq) / nq, (gqn.y - qn.y * dq) / nq, (gqn.z - qn.z * dq) / nq,
+                          (gqn.w - qn.w * dq) / nq);
+    }
+    gp_out = gup;  // d up / d pc = I
+    gq_out = guq;
+    const float g_sp = gup.x * tu.x + gup.y * tu.y;
+    const float g_sq = guq.x * gV.x + guq.y * gV.y + guq.z * gV.z + guq.w * gV.w;
+    gpa1 += g_sp * sigma / eta_p;
+    gpa0 += g_sq * sigma / eta_q;
+    const float2 gtu = make_float2(sp * gup.x, sp * gup.y);
+    ga += gtu.x * gU.x;
+    gb += gtu.y * gU.y;
+    gc += gtu.x * gU.y + gtu.y * gU.x;
+    gabc[0] += ga;
+    gabc[1] += gb;
+    gabc[2] += gc;
+    const float2 ggU = make_float2(a * gtu.x + c * gtu.y, c * gtu.x + b * gtu.y);
+    const float4 ggV = make_float4(sq * guq.x, sq * guq.y, sq * guq.z, sq * guq.w);
+    // gU.x = sx (uX - uc) - vc.x ; gU.y = sy (uY - uc) - vc.y
+    // gV.x = sx (vX.x - vc.x) ; gV.w = sx (vX.y - vc.y) ; gV.y = sy (vY.y - vc.y) ; gV.z = sy (vY.x - vc.x)
+    msg.um[0] = -sx * ggU.x - sy * ggU.y;
+    msg.um[1] = sx * ggU.x;
+    msg.um[2] = sy * ggU.y;
+    msg.vm[0] = -ggU.x - sx * ggV.x - sy * ggV.z;
+    msg.vm[1] = -ggU.y - sx * ggV.w - sy * ggV.y;
+    msg.vm[2] = sx * ggV.x;
+    msg.vm[3] = sx * ggV.w;
+    msg.vm[4] = sy * ggV.z;
+    msg.vm[5] = sy * ggV.y;
+}
+
+}  // namespace vfs

@@ -14,14 +14,11 @@
 // when the norm exceeds 1; max(exp, 1e-8) differentiates exp when exp >= 1e-8;
 // min(d, limit) passes the gradient to d when d < limit; the threshold uses
 // the branch its forward took; the median routes to the selected element.
-#include "vfs_device.h"
-#include "vfs_pd.h"
+#include "vfs_adj_pixel.h"
 
 using namespace vfs;
 
 namespace {
-
-enum { GP_LAMBDA = 0, GP_ALPHA0 = 1, GP_ALPHA1 = 2, GP_BETA = 3, GP_GAMMA = 4, GP_N = 5 };
 
 #define VFS_ADJ_PIXEL(H, W)                              \
     const int X = blockIdx.x * blockDim.x + threadIdx.x; \
@@ -29,78 +26,29 @@ enum { GP_LAMBDA = 0, GP_ALPHA0 = 1, GP_ALPHA1 = 2, GP_BETA = 3, GP_GAMMA = 4, G
     if (X >= (W) || Y >= (H)) return;                    \
     const int idx = Y * (W) + X;
 
-struct Nbr {
-    int iE, iW, iS, iN;
-    bool m, mE, mW, mS, mN;
+// dual^T messages stored per pixel: um (H,W,3), vm (H,W,6)
+struct GlobalMsgs {
+    const float* um;
+    const float* vm;
+    int W;
+    __device__ __forceinline__ DualMsg operator()(int x, int y) const
+    {
+        const int i = y * W + x;
+        DualMsg m;
+        for (int k = 0; k < 3; ++k) m.um[k] = um[3 * i + k];
+        for (int k = 0; k < 6; ++k) m.vm[k] = vm[6 * i + k];
+        return m;
+    }
 };
 
-__device__ __forceinline__ Nbr nbr(const uint8_t* __restrict__ mask, int X, int Y, int H, int W)
-{
-    Nbr n;
-    n.iE = Y * W + clamp_idx(X + 1, W);
-    n.iW = Y * W + clamp_idx(X - 1, W);
-    n.iS = clamp_idx(Y + 1, H) * W + X;
-    n.iN = clamp_idx(Y - 1, H) * W + X;
-    n.m = mask[Y * W + X] != 0;
-    n.mE = mask[n.iE] != 0;
-    n.mW = mask[n.iW] != 0;
-    n.mS = mask[n.iS] != 0;
-    n.mN = mask[n.iN] != 0;
-    return n;
-}
-
-// x / y neighbour whose u_bar, v_bar the dual step of pixel (X, Y) reads
-__device__ __forceinline__ int dual_x_target(const uint8_t* __restrict__ mask, int X, int Y, int W)
-{
-    const int iE = Y * W + clamp_idx(X + 1, W);
-    return mask[iE] != 0 ? iE : Y * W + clamp_idx(X - 1, W);
-}
-
-__device__ __forceinline__ int dual_y_target(const uint8_t* __restrict__ mask, int X, int Y, int H, int W)
-{
-    const int iS = clamp_idx(Y + 1, H) * W + X;
-    return mask[iS] != 0 ? iS : clamp_idx(Y - 1, H) * W + X;
-}
-
-// Collect the u_bar / v_bar messages of the dual adjoint addressed to pixel (X, Y):
-// um = (own, to x-target, to y-target); vm = (own.x, own.y, x-target.x, x-target.y,
-// y-target.x, y-target.y).
-__device__ __forceinline__ void gather_dual_msgs(const uint8_t* __restrict__ mask, const float* __restrict__ um,
-                                                 const float* __restrict__ vm, int X, int Y, int H, int W,
-                                                 float& gub, float2& gvb)
-{
-    const int j = Y * W + X;
-    gub += um[3 * j];
-    gvb.x += vm[6 * j];
-    gvb.y += vm[6 * j + 1];
-    for (int dx = -1; dx <= 1; ++dx) {
-        const int xs = X + dx;
-        if (xs < 0 || xs >= W) continue;
-        const int i = Y * W + xs;
-        if (mask[i] == 0 || dual_x_target(mask, xs, Y, W) != j) continue;
-        gub += um[3 * i + 1];
-        gvb.x += vm[6 * i + 2];
-        gvb.y += vm[6 * i + 3];
-    }
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int ys = Y + dy;
-        if (ys < 0 || ys >= H) continue;
-        const int i = ys * W + X;
-        if (mask[i] == 0 || dual_y_target(mask, X, ys, H, W) != j) continue;
-        gub += um[3 * i + 2];
-        gvb.x += vm[6 * i + 4];
-        gvb.y += vm[6 * i + 5];
-    }
-}
-
 // ---------------------------------------------------------------------------
-// primal-dual iteration, reverse
+// primal-dual iteration, reverse (one kernel per step)
 // ---------------------------------------------------------------------------
 
-// Reverse of updatePrimalVariables (Python/vfs.cu:507-570) at iteration t.
+// Reverse of updatePrimalVariables at iteration t.
 // In:  gubB / gvbB hold the own-pixel part of the adjoint of u_bar_t / v_bar_t; the
-//      dual messages um / vm of the following iteration complete it (pass
-//      has_msgs = 0 when there is none).
+//      dual messages um / vm of the following iteration complete it (has_msgs = 0
+//      when there is none).
 // Out: gubB / gvbB := own-pixel part of the adjoint of u_bar_{t-1} / v_bar_{t-1};
 //      gp += , gq += (own), qmsg = (to W: x, w; to N: y, z), gu += (adjoint of u_t);
 //      per-pixel accumulators of abc, eta and the parameters.
@@ -110,17 +58,14 @@ __global__ void __launch_bounds__(256) k_adj_primal(float* __restrict__ gubB, fl
                                                     float4* __restrict__ qmsg, float* __restrict__ gu,
                                                     float* __restrict__ gabc, float* __restrict__ geta,
                                                     float* __restrict__ gpar, const float2* __restrict__ p,
-                                                    const float4* __restrict__ q, const float* __restrict__ u,
-                                                    const float* __restrict__ u_bar_prev,
-                                                    const float2* __restrict__ v_bar_prev,
-                                                    const float* __restrict__ abc, const float* __restrict__ eta,
-                                                    const uint8_t* __restrict__ mask, int H, int W, float tau,
-                                                    float mu, float alpha0, float alpha1)
+                                                    const float4* __restrict__ q, const float* __restrict__ abc,
+                                                    const float* __restrict__ eta, const uint8_t* __restrict__ mask,
+                                                    int H, int W, float tau, float mu, float alpha0, float alpha1)
 {
     VFS_ADJ_PIXEL(H, W)
     float g_ubo = gubB[idx];
     float2 g_vbo = gvbB[idx];
-    if (has_msgs) gather_dual_msgs(mask, um, vm, X, Y, H, W, g_ubo, g_vbo);
+    if (has_msgs) gather_msgs(mask, GlobalMsgs{um, vm, W}, X, Y, H, W, g_ubo, g_vbo);
     const Nbr n = nbr(mask, X, Y, H, W);
     if (!n.m) {
         // outputs are constant zero: nothing flows back
@@ -129,73 +74,31 @@ __global__ void __launch_bounds__(256) k_adj_primal(float* __restrict__ gubB, fl
         qmsg[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    const float4 qc = q[idx];
     const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 qW = n.mW ? q[n.iW] : zero4;
-    const float4 qN = n.mN ? q[n.iN] : zero4;
-    float4 gQ = zero4;
-    if (n.mE) {
-        gQ.x = qc.x;
-        gQ.w = qc.w;
-    }
-    if (n.mW) {
-        gQ.x -= qW.x;
-        gQ.w -= qW.w;
-    }
-    if (n.mS) {
-        gQ.y = qc.y;
-        gQ.z = qc.z;
-    }
-    if (n.mN) {
-        gQ.y -= qN.y;
-        gQ.z -= qN.z;
-    }
-    const float a = abc[3 * idx], b = abc[3 * idx + 1], c = abc[3 * idx + 2];
-    const float2 pc = p[idx];
-    const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
-    const float e1 = eta[3 * idx + 1], e2 = eta[3 * idx + 2];
-    const bool d1 = !(fabsf(e1) < kEps), d2 = !(fabsf(e2) < kEps);
-    const float t1 = d1 ? tau / e1 : tau;
-    const float t2 = d2 ? tau / e2 : tau;
-    const float S1 = gQ.x + gQ.z, S2 = gQ.w + gQ.y;
-    const float A1 = fmaf(alpha1, tu.x, alpha0 * S1), A2 = fmaf(alpha1, tu.y, alpha0 * S2);
-    // ub_out = (1 + mu) u - mu ub ; vb_out = (1 + mu) v - mu vb ; v = t A + vb
-    const float g_u = (1.0f + mu) * g_ubo;
-    const float g_vx = (1.0f + mu) * g_vbo.x, g_vy = (1.0f + mu) * g_vbo.y;
-    gubB[idx] = -mu * g_ubo;
-    gvbB[idx] = make_float2(g_vx - mu * g_vbo.x, g_vy - mu * g_vbo.y);
-    gu[idx] += g_u;
-    const float g_A1 = g_vx * t1, g_A2 = g_vy * t2;
-    const float g_t1 = g_vx * A1, g_t2 = g_vy * A2;
+    float ubB, uu = gu[idx];
+    float2 vbB, pp = gp[idx];
+    float4 qq = gq[idx], qm;
     float* gpp = gpar + GP_N * idx;
-    gpp[GP_ALPHA1] += g_A1 * tu.x + g_A2 * tu.y;
-    gpp[GP_ALPHA0] += g_A1 * S1 + g_A2 * S2;
-    if (d1) geta[3 * idx + 1] += -g_t1 * tau / (e1 * e1);
-    if (d2) geta[3 * idx + 2] += -g_t2 * tau / (e2 * e2);
-    const float2 g_tu = make_float2(g_A1 * alpha1, g_A2 * alpha1);
-    float2 gpc = gp[idx];
-    gpc.x += a * g_tu.x + c * g_tu.y;
-    gpc.y += c * g_tu.x + b * g_tu.y;
-    gp[idx] = gpc;
-    gabc[3 * idx] += g_tu.x * pc.x;
-    gabc[3 * idx + 1] += g_tu.y * pc.y;
-    gabc[3 * idx + 2] += g_tu.x * pc.y + g_tu.y * pc.x;
-    const float g_S1 = g_A1 * alpha0, g_S2 = g_A2 * alpha0;
-    float4 gqc = gq[idx];
-    if (n.mE) {
-        gqc.x += g_S1;
-        gqc.w += g_S2;
-    }
-    if (n.mS) {
-        gqc.z += g_S1;
-        gqc.y += g_S2;
-    }
-    gq[idx] = gqc;
-    // messages: to W (x, w components of q_W), to N (y, z components of q_N)
-    qmsg[idx] = make_float4(n.mW ? -g_S1 : 0.0f, n.mW ? -g_S2 : 0.0f, n.mN ? -g_S2 : 0.0f, n.mN ? -g_S1 : 0.0f);
+    float ga[3] = {gabc[3 * idx], gabc[3 * idx + 1], gabc[3 * idx + 2]};
+    float e1 = geta[3 * idx + 1], e2 = geta[3 * idx + 2];
+    float pa0 = gpp[GP_ALPHA0], pa1 = gpp[GP_ALPHA1];
+    primal_adj_pixel(n, q[idx], n.mW ? q[n.iW] : zero4, n.mN ? q[n.iN] : zero4, abc[3 * idx], abc[3 * idx + 1],
+                     abc[3 * idx + 2], p[idx], eta[3 * idx + 1], eta[3 * idx + 2], tau, mu, alpha0, alpha1, g_ubo,
+                     g_vbo, ubB, vbB, uu, pp, qq, qm, ga, e1, e2, pa0, pa1);
+    gubB[idx] = ubB;
+    gvbB[idx] = vbB;
+    gu[idx] = uu;
+    gpp[GP_ALPHA1] = pa1;
+    gpp[GP_ALPHA0] = pa0;
+    geta[3 * idx + 1] = e1;
+    geta[3 * idx + 2] = e2;
+    gp[idx] = pp;
+    for (int k = 0; k < 3; ++k) gabc[3 * idx + k] = ga[k];
+    gq[idx] = qq;
+    qmsg[idx] = qm;
 }
 
-// Reverse of l1Thresholding (Python/vfs.cu:421-495) at iteration t.
+// Reverse of l1Thresholding at iteration t.
 // In:  gu = adjoint of u_t (complete).  State: p_t (tp_t = T p_t is recomputed),
 //      u_{t-1}, u_bar_{t-1}.
 // Out: gu := adjoint of u_{t-1}; tmsg = (own tp.x, own tp.y, to W (x), to N (y));
@@ -211,88 +114,27 @@ __global__ void __launch_bounds__(256) k_adj_l1(float* __restrict__ gu, float4* 
 {
     VFS_ADJ_PIXEL(H, W)
     const Nbr n = nbr(mask, X, Y, H, W);
-    const float gout = gu[idx];
     if (!n.m) {
         gu[idx] = 0.0f;
         tmsg[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    auto tp_at = [&](int i) {
-        if (mask[i] == 0) return make_float2(0.0f, 0.0f);
-        const float2 pp = p[i];
-        const float a = abc[3 * i], b = abc[3 * i + 1], c = abc[3 * i + 2];
-        return make_float2(fmaf(a, pp.x, c * pp.y), fmaf(c, pp.x, b * pp.y));
-    };
-    const float2 tc = tp_at(idx);
-    const float tpWx = tp_at(n.iW).x, tpNy = tp_at(n.iN).y;
-    float cxc, cxw, cyc, cyn;
-    if (n.mE && n.mW) {
-        cxc = 1.0f;
-        cxw = -1.0f;
-    } else if (!n.mE) {
-        cxc = 0.0f;
-        cxw = -1.0f;
-    } else {
-        cxc = 1.0f;
-        cxw = 0.0f;
-    }
-    if (n.mS && n.mN) {
-        cyc = 1.0f;
-        cyn = -1.0f;
-    } else if (!n.mS) {
-        cyc = 0.0f;
-        cyn = -1.0f;
-    } else {
-        cyc = 1.0f;
-        cyn = 0.0f;
-    }
-    const float gx = (n.mE && n.mW) ? tc.x - tpWx : (!n.mE ? -tpWx : tc.x);
-    const float gy = (n.mS && n.mN) ? tc.y - tpNy : (!n.mS ? -tpNy : tc.y);
-    const float2 d = deriv[idx];
-    const float dIu = d.x, dIz = d.y;
-    const float e0 = eta[3 * idx];
-    const bool de = !(fabsf(e0) < kEps);
-    const float te = de ? tau / e0 : tau;
-    const float uc = u_prev[idx], ubc = u_bar_prev[idx];
-    const float ut = ubc + te * (gx + gy);
-    const float rho = dIu * (ut - uc) + dIz;
-    const float bound = lambda * te * dIu * dIu;
-    const float d0 = ut - uc;
-    float g_d0 = 0.0f, g_dIu = 0.0f, g_dIz = 0.0f, g_te = 0.0f, g_lam = 0.0f;
-    if ((rho <= bound) && (rho >= -bound)) {
-        if (fabsf(dIu) < kEps) {
-            g_d0 = gout;
-        } else {  // du = d0 - rho / dIu
-            const float g_rho = -gout / dIu;
-            g_d0 = gout + g_rho * dIu;
-            g_dIu = gout * rho / (dIu * dIu) + g_rho * d0;
-            g_dIz = g_rho;
-        }
-    } else if (rho < -bound) {  // du = d0 + lambda te dIu
-        g_d0 = gout;
-        g_lam = gout * te * dIu;
-        g_te = gout * lambda * dIu;
-        g_dIu = gout * lambda * te;
-    } else if (rho > bound) {  // du = d0 - lambda te dIu
-        g_d0 = gout;
-        g_lam = -gout * te * dIu;
-        g_te = -gout * lambda * dIu;
-        g_dIu = -gout * lambda * te;
-    }
-    // out = uc + du, d0 = ut - uc, ut = ubc + te (gx + gy)
-    gu[idx] = gout - g_d0;
-    const float g_ut = g_d0;
-    gubB[idx] += g_ut;
-    g_te += g_ut * (gx + gy);
-    const float g_g = g_ut * te;
-    if (de) geta[3 * idx] += -g_te * tau / (e0 * e0);
-    gderiv[2 * idx] += g_dIu;
-    gderiv[2 * idx + 1] += g_dIz;
-    gpar[GP_N * idx + GP_LAMBDA] += g_lam;
-    tmsg[idx] = make_float4(cxc * g_g, cyc * g_g, cxw * g_g, cyn * g_g);
+    float4 tm;
+    float ubB = gubB[idx], e0 = geta[3 * idx], lam = gpar[GP_N * idx + GP_LAMBDA];
+    float2 gd = make_float2(gderiv[2 * idx], gderiv[2 * idx + 1]);
+    const float g = l1_adj_pixel(n, gu[idx], tp_at(p, abc, mask, idx), tp_at(p, abc, mask, n.iW).x,
+                                 tp_at(p, abc, mask, n.iN).y, deriv[idx], eta[3 * idx], u_prev[idx], u_bar_prev[idx],
+                                 tau, lambda, tm, ubB, e0, gd, lam);
+    gu[idx] = g;
+    gubB[idx] = ubB;
+    geta[3 * idx] = e0;
+    gderiv[2 * idx] = gd.x;
+    gderiv[2 * idx + 1] = gd.y;
+    gpar[GP_N * idx + GP_LAMBDA] = lam;
+    tmsg[idx] = tm;
 }
 
-// Reverse of updateDualVariables (Python/vfs.cu:326-409) at iteration t.
+// Reverse of updateDualVariables at iteration t.
 // In:  gp, gq: adjoints of p_t, q_t (own parts; the primal messages qmsg of this
 //      iteration complete gq); tmsg: threshold messages (complete gtp_t).
 //      State: p_{t-1}, q_{t-1}, u_bar_{t-1}, v_bar_{t-1}.
@@ -311,40 +153,9 @@ __global__ void __launch_bounds__(256) k_adj_dual(float2* __restrict__ gp, float
 {
     VFS_ADJ_PIXEL(H, W)
     const Nbr n = nbr(mask, X, Y, H, W);
-    // complete the adjoints of q_t and tp_t at this pixel
     float4 gqn = gq[idx];
-    {
-        const float4 m0 = qmsg[idx];
-        // senders to W: self (X = 0) or the east neighbour; to N: self (Y = 0) or the south neighbour
-        if (n.iW == idx) {
-            gqn.x += m0.x;
-            gqn.w += m0.y;
-        }
-        if (X + 1 < W) {
-            const float4 me = qmsg[idx + 1];
-            gqn.x += me.x;
-            gqn.w += me.y;
-        }
-        if (n.iN == idx) {
-            gqn.y += m0.z;
-            gqn.z += m0.w;
-        }
-        if (Y + 1 < H) {
-            const float4 ms = qmsg[idx + W];
-            gqn.y += ms.z;
-            gqn.z += ms.w;
-        }
-    }
     float2 gtp;
-    {
-        const float4 t0 = tmsg[idx];
-        gtp = make_float2(t0.x, t0.y);
-        if (n.iW == idx) gtp.x += t0.z;
-        if (X + 1 < W) gtp.x += tmsg[idx + 1].z;
-        if (n.iN == idx) gtp.y += t0.w;
-        if (Y + 1 < H) gtp.y += tmsg[idx + W].w;
-    }
-    float2 gpn = gp[idx];
+    collect_qt(qmsg, tmsg, n, X, Y, H, W, idx, gqn, gtp);
     if (!n.m) {
         gp[idx] = make_float2(0.0f, 0.0f);
         gq[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -354,88 +165,22 @@ __global__ void __launch_bounds__(256) k_adj_dual(float2* __restrict__ gp, float
     }
     const bool fx = n.mE, fy = n.mS;
     const int ix = fx ? n.iE : n.iW, iy = fy ? n.iS : n.iN;
-    const float sx = fx ? 1.0f : -1.0f, sy = fy ? 1.0f : -1.0f;
-    const float uc = u_bar_prev[idx];
-    const float2 vc = v_bar_prev[idx];
-    const float uX = u_bar_prev[ix], uY = u_bar_prev[iy];
-    const float2 vX = v_bar_prev[ix], vY = v_bar_prev[iy];
-    float2 gU;
-    float4 gV;
-    if (fx) {
-        gU.x = uX - uc - vc.x;
-        gV.x = vX.x - vc.x;
-        gV.w = vX.y - vc.y;
-    } else {
-        gU.x = uc - uX - vc.x;
-        gV.x = vc.x - vX.x;
-        gV.w = vc.y - vX.y;
-    }
-    if (fy) {
-        gU.y = uY - uc - vc.y;
-        gV.y = vY.y - vc.y;
-        gV.z = vY.x - vc.x;
-    } else {
-        gU.y = uc - uY - vc.y;
-        gV.y = vc.y - vY.y;
-        gV.z = vc.x - vY.x;
-    }
-    const float a = abc[3 * idx], b = abc[3 * idx + 1], c = abc[3 * idx + 2];
-    const float sp = dual_scale(alpha1, sigma, eta_p), sq = dual_scale(alpha0, sigma, eta_q);
-    const float2 pc = p_prev[idx];
-    const float4 qc = q_prev[idx];
-    const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
-    const float2 up = make_float2(fmaf(sp, tu.x, pc.x), fmaf(sp, tu.y, pc.y));
-    const float np_ = hypotf(up.x, up.y);
-    const bool projp = fmaxf(np_, 1.0f) > 1.0f;
-    const float2 pn = projp ? make_float2(up.x / np_, up.y / np_) : up;
-    const float4 uq = make_float4(fmaf(sq, gV.x, qc.x), fmaf(sq, gV.y, qc.y), fmaf(sq, gV.z, qc.z),
-                                  fmaf(sq, gV.w, qc.w));
-    const float nq = norm4df(uq.x, uq.y, uq.z, uq.w);
-    const bool projq = fmaxf(nq, 1.0f) > 1.0f;
-    const float4 qn = projq ? make_float4(uq.x / nq, uq.y / nq, uq.z / nq, uq.w / nq) : uq;
-    // tp = T pn
-    float ga = gtp.x * pn.x, gb = gtp.y * pn.y, gc = gtp.x * pn.y + gtp.y * pn.x;
-    gpn.x += a * gtp.x + c * gtp.y;
-    gpn.y += c * gtp.x + b * gtp.y;
-    // projection of p
-    float2 gup = gpn;
-    if (projp) {
-        const float dp = pn.x * gpn.x + pn.y * gpn.y;
-        gup = make_float2((gpn.x - pn.x * dp) / np_, (gpn.y - pn.y * dp) / np_);
-    }
-    float4 guq = gqn;
-    if (projq) {
-        const float dq = qn.x * gqn.x + qn.y * gqn.y + qn.z * gqn.z + qn.w * gqn.w;
-        guq = make_float4((gqn.x - qn.x * dq) / nq, (gqn.y - qn.y * dq) / nq, (gqn.z - qn.z * dq) / nq,
-                          (gqn.w - qn.w * dq) / nq);
-    }
-    gp[idx] = gup;  // d up / d pc = I
-    gq[idx] = guq;
-    const float g_sp = gup.x * tu.x + gup.y * tu.y;
-    const float g_sq = guq.x * gV.x + guq.y * gV.y + guq.z * gV.z + guq.w * gV.w;
+    float2 gpo;
+    float4 gqo;
+    DualMsg msg;
+    float ga[3] = {gabc[3 * idx], gabc[3 * idx + 1], gabc[3 * idx + 2]};
     float* gpp = gpar + GP_N * idx;
-    gpp[GP_ALPHA1] += g_sp * sigma / eta_p;
-    gpp[GP_ALPHA0] += g_sq * sigma / eta_q;
-    const float2 gtu = make_float2(sp * gup.x, sp * gup.y);
-    ga += gtu.x * gU.x;
-    gb += gtu.y * gU.y;
-    gc += gtu.x * gU.y + gtu.y * gU.x;
-    gabc[3 * idx] += ga;
-    gabc[3 * idx + 1] += gb;
-    gabc[3 * idx + 2] += gc;
-    const float2 ggU = make_float2(a * gtu.x + c * gtu.y, c * gtu.x + b * gtu.y);
-    const float4 ggV = make_float4(sq * guq.x, sq * guq.y, sq * guq.z, sq * guq.w);
-    // gU.x = sx (uX - uc) - vc.x ; gU.y = sy (uY - uc) - vc.y
-    // gV.x = sx (vX.x - vc.x) ; gV.w = sx (vX.y - vc.y) ; gV.y = sy (vY.y - vc.y) ; gV.z = sy (vY.x - vc.x)
-    um[3 * idx + 0] = -sx * ggU.x - sy * ggU.y;
-    um[3 * idx + 1] = sx * ggU.x;
-    um[3 * idx + 2] = sy * ggU.y;
-    vm[6 * idx + 0] = -ggU.x - sx * ggV.x - sy * ggV.z;
-    vm[6 * idx + 1] = -ggU.y - sx * ggV.w - sy * ggV.y;
-    vm[6 * idx + 2] = sx * ggV.x;
-    vm[6 * idx + 3] = sx * ggV.w;
-    vm[6 * idx + 4] = sy * ggV.z;
-    vm[6 * idx + 5] = sy * ggV.y;
+    float pa0 = gpp[GP_ALPHA0], pa1 = gpp[GP_ALPHA1];
+    dual_adj_pixel(fx, fy, u_bar_prev[idx], v_bar_prev[idx], u_bar_prev[ix], v_bar_prev[ix], u_bar_prev[iy],
+                   v_bar_prev[iy], abc[3 * idx], abc[3 * idx + 1], abc[3 * idx + 2], p_prev[idx], q_prev[idx], sigma,
+                   alpha0, alpha1, eta_p, eta_q, gp[idx], gqn, gtp, gpo, gqo, msg, ga, pa0, pa1);
+    gp[idx] = gpo;
+    gq[idx] = gqo;
+    gpp[GP_ALPHA1] = pa1;
+    gpp[GP_ALPHA0] = pa0;
+    for (int k = 0; k < 3; ++k) gabc[3 * idx + k] = ga[k];
+    for (int k = 0; k < 3; ++k) um[3 * idx + k] = msg.um[k];
+    for (int k = 0; k < 6; ++k) vm[6 * idx + k] = msg.vm[k];
 }
 
 // Adjoints of u_bar and v_bar at the start of the primal-dual loop: own parts +
@@ -448,7 +193,7 @@ __global__ void __launch_bounds__(256) k_adj_pd_finish(float* __restrict__ gub, 
     VFS_ADJ_PIXEL(H, W)
     float g = gubB[idx];
     float2 gv = gvbB[idx];
-    gather_dual_msgs(mask, um, vm, X, Y, H, W, g, gv);
+    gather_msgs(mask, GlobalMsgs{um, vm, W}, X, Y, H, W, g, gv);
     gub[idx] = g;
     gvb[idx] = gv;
 }
@@ -858,9 +603,8 @@ int vfs_adj_pd_iteration(void* stream, const vfs_adj_pd* a, const vfs_params* pr
     hipLaunchKernelGGL(k_adj_primal, grid2d(H, W), block2d(), 0, s, a->gub, reinterpret_cast<float2*>(a->gvb), a->um,
                        a->vm, has_msgs, reinterpret_cast<float2*>(a->gp), reinterpret_cast<float4*>(a->gq),
                        reinterpret_cast<float4*>(a->qmsg), a->gu, a->gabc, a->geta, a->gpar,
-                       reinterpret_cast<const float2*>(a->p), reinterpret_cast<const float4*>(a->q), a->u,
-                       a->u_bar_prev, reinterpret_cast<const float2*>(a->v_bar_prev), a->abc, a->eta, a->mask, H, W,
-                       tau, mu, prm->alpha0, prm->alpha1);
+                       reinterpret_cast<const float2*>(a->p), reinterpret_cast<const float4*>(a->q), a->abc, a->eta,
+                       a->mask, H, W, tau, mu, prm->alpha0, prm->alpha1);
     hipLaunchKernelGGL(k_adj_l1, grid2d(H, W), block2d(), 0, s, a->gu, reinterpret_cast<float4*>(a->tmsg), a->gub,
                        a->gderiv, a->geta, a->gpar, reinterpret_cast<const float2*>(a->p), a->abc, a->u_prev,
                        a->u_bar_prev, reinterpret_cast<const float2*>(a->deriv), a->eta, a->mask, H, W, tau,

@@ -1,0 +1,756 @@
+// Native level driver of the adjoint: the forward warp loop of one level that also
+// records what the reverse sweep needs, and the whole reverse sweep of one level.
+//
+// Save area of a level (caller-allocated, vfs_adj_save_bytes): for every warp the
+// warp-start u, u_bar and flow, then ceil(n_iter / ckpt) checkpoints of the
+// primal-dual state (p, q, u, u_bar, v_bar) taken before iterations 0, ckpt,
+// 2 ckpt, ...  Workspace (vfs_adj_workspace_bytes): a tape of ckpt states that
+// one checkpoint segment is recomputed into, and the adjoint buffers.  Every
+// array starts on a 256-byte boundary, so the float2 / float4 views the
+// kernels take stay aligned for any H x W.
+//
+// Everything is enqueued on one stream without host synchronisation and
+// without memsets (clearing is done by a fill kernel), so both functions can
+// be recorded into a HIP graph by stream capture.
+#include "vfs_adj_pixel.h"
+
+using namespace vfs;
+
+namespace {
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int64_t kAlign = 64;  // floats (256 bytes)
+inline int64_t al(int64_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+struct State {
+    float *p, *q, *u, *ub, *vb;
+};
+
+inline int64_t state_floats(int64_t n) { return al(2 * n) + al(4 * n) + 2 * al(n) + al(2 * n); }
+
+inline State state_at(float* base, int64_t n)
+{
+    State s;
+    s.p = base;
+    s.q = s.p + al(2 * n);
+    s.u = s.q + al(4 * n);
+    s.ub = s.u + al(n);
+    s.vb = s.ub + al(n);
+    return s;
+}
+
+inline int n_ckpt(int n_iter, int ck) { return n_iter <= 0 ? 0 : (n_iter + ck - 1) / ck; }
+
+// per warp: u, u_bar, flow, then the checkpoints
+inline int64_t warp_rec_floats(int64_t n) { return 2 * al(n) + al(2 * n); }
+inline int64_t warp_save_floats(int64_t n, int n_iter, int ck)
+{
+    return warp_rec_floats(n) + static_cast<int64_t>(n_ckpt(n_iter, ck)) * state_floats(n);
+}
+
+struct WarpSave {
+    float *u, *ub, *flow;
+    float* ckpt0;
+};
+
+inline WarpSave warp_save_at(float* save, int64_t n, int n_iter, int ck, int w)
+{
+    WarpSave r;
+    r.u = save + static_cast<int64_t>(w) * warp_save_floats(n, n_iter, ck);
+    r.ub = r.u + al(n);
+    r.flow = r.ub + al(n);
+    r.ckpt0 = r.flow + al(2 * n);
+    return r;
+}
+
+// adjoint workspace (after the tape)
+struct AdjWs {
+    float* tp;  // forward recompute's T p
+    float *gp, *gq, *um, *vm, *qmsg, *tmsg, *gabc, *geta, *gderiv, *gvbB, *gubB, *guf, *gtv, *gow, *guth, *gvb_fin,
+        *gIs;
+    int* pos;
+    // second copies: a fused launch reads one and writes the other (its ring reads
+    // a neighbouring tile's values, which that tile's launch also rewrites)
+    float *gp2, *gq2, *qmsg2, *tmsg2;
+    double* gpart;
+    float* tape;
+};
+
+// fused sweep tile (one 256-thread block) and its per-block parameter slots
+constexpr int FT_X = 32, FT_Y = 8;
+inline int64_t fused_blocks(int H, int W)
+{
+    return static_cast<int64_t>((W + FT_X - 1) / FT_X) * ((H + FT_Y - 1) / FT_Y);
+}
+
+inline int64_t adj_ws_floats(int64_t n, int64_t nblocks)
+{
+    // tp 2, gp 2, gq 4, um 3, vm 6, qmsg 4, tmsg 4, gabc 3, geta 3, gderiv 2, gvbB 2, gubB 1, guf 1, gtv 2,
+    // gow 1, guth 1, gvb_fin 2, gIs 1, pos 1, and the second copies gp 2, gq 4, qmsg 4, tmsg 4
+    const int ch[] = {2, 2, 4, 3, 6, 4, 4, 3, 3, 2, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 4, 4, 4};
+    int64_t t = 0;
+    for (int c : ch) t += al(c * n);
+    return t + al(2 * 3 * nblocks);  // + (lambda, alpha0, alpha1) per block in double
+}
+
+inline AdjWs adj_ws_at(float* base, int64_t n, int64_t nblocks)
+{
+    AdjWs a;
+    float* c = base;
+    auto take = [&](int ch) {
+        float* r = c;
+        c += al(ch * n);
+        return r;
+    };
+    a.tp = take(2);
+    a.gp = take(2);
+    a.gq = take(4);
+    a.um = take(3);
+    a.vm = take(6);
+    a.qmsg = take(4);
+    a.tmsg = take(4);
+    a.gabc = take(3);
+    a.geta = take(3);
+    a.gderiv = take(2);
+    a.gvbB = take(2);
+    a.gubB = take(1);
+    a.guf = take(1);
+    a.gtv = take(2);
+    a.gow = take(1);
+    a.guth = take(1);
+    a.gvb_fin = take(2);
+    a.gIs = take(1);
+    a.pos = reinterpret_cast<int*>(take(1));
+    a.gp2 = take(2);
+    a.gq2 = take(4);
+    a.qmsg2 = take(4);
+    a.tmsg2 = take(4);
+    a.gpart = reinterpret_cast<double*>(c);
+    c += al(2 * 3 * nblocks);
+    a.tape = c;
+    return a;
+}
+
+// updateDualVariables (Python/vfs.cu:326-419) reading (p, q) from one state and
+// writing them to another; the per-pixel arithmetic is the shared dual_pixel, so
+// a recomputed iteration reproduces the in-place kernel's values.
+__global__ void __launch_bounds__(256) k_dual_oop(float2* __restrict__ p_out, float2* __restrict__ tp,
+                                                  float4* __restrict__ q_out, const float2* __restrict__ p_in,
+                                                  const float4* __restrict__ q_in, float sigma,
+                                                  const float* __restrict__ u, const float2* __restrict__ v,
+                                                  const float* __restrict__ abc, const uint8_t* __restrict__ mask,
+                                                  int H, int W, float alpha0, float alpha1, float eta_p, float eta_q)
+{
+    const int X = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (X >= W || Y >= H) return;
+    const int idx = Y * W + X;
+    if (mask[idx] == 0) {
+        p_out[idx] = make_float2(0.0f, 0.0f);
+        tp[idx] = make_float2(0.0f, 0.0f);
+        q_out[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const int iE = Y * W + clamp_idx(X + 1, W), iW = Y * W + clamp_idx(X - 1, W);
+    const int iS = clamp_idx(Y + 1, H) * W + X, iN = clamp_idx(Y - 1, H) * W + X;
+    const bool fx = mask[iE] != 0, fy = mask[iS] != 0;
+    const int ix = fx ? iE : iW, iy = fy ? iS : iN;
+    float2 pn, tpn;
+    float4 qn;
+    dual_pixel(u[idx], v[idx], u[ix], v[ix], fx, u[iy], v[iy], fy, abc[3 * idx + 0], abc[3 * idx + 1],
+               abc[3 * idx + 2], p_in[idx], q_in[idx], dual_scale(alpha1, sigma, eta_p),
+               dual_scale(alpha0, sigma, eta_q), pn, tpn, qn);
+    p_out[idx] = pn;
+    tp[idx] = tpn;
+    q_out[idx] = qn;
+}
+
+// Per-warp reset + warp-start record: p = q = v_bar = 0, u0 = u, and the saved
+// copies of u, u_bar and flow.
+__global__ void __launch_bounds__(256) k_warp_reset_save(float2* __restrict__ p, float4* __restrict__ q,
+                                                         float2* __restrict__ v_bar, float* __restrict__ u0,
+                                                         const float* __restrict__ u, const float* __restrict__ u_bar,
+                                                         const float2* __restrict__ flow, float* __restrict__ su,
+                                                         float* __restrict__ sub, float2* __restrict__ sflow, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = make_float2(0.0f, 0.0f);
+    v_bar[i] = make_float2(0.0f, 0.0f);
+    q[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float uu = u[i];
+    u0[i] = uu;
+    su[i] = uu;
+    sub[i] = u_bar[i];
+    sflow[i] = flow[i];
+}
+
+// Checkpoint of the primal-dual state.
+__global__ void __launch_bounds__(256) k_save_state(float2* __restrict__ sp, float4* __restrict__ sq,
+                                                    float* __restrict__ su, float* __restrict__ sub,
+                                                    float2* __restrict__ svb, const float2* __restrict__ p,
+                                                    const float4* __restrict__ q, const float* __restrict__ u,
+                                                    const float* __restrict__ ub, const float2* __restrict__ vb, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sp[i] = p[i];
+    sq[i] = q[i];
+    su[i] = u[i];
+    sub[i] = ub[i];
+    svb[i] = vb[i];
+}
+
+// Same as the solver's clamp: flow += delta tv, u = u0 + delta (Python/vfs.cu:641-676,
+// Python/vfs.py:519-521).
+__global__ void __launch_bounds__(256) k_flow_clamp_u(float2* __restrict__ flow, float* __restrict__ u,
+                                                      const float* __restrict__ u0, const float* __restrict__ uf,
+                                                      const float2* __restrict__ tv, const uint8_t* __restrict__ mask,
+                                                      int n, float limit)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float base = u0[i];
+    if (mask[i] == 0) {
+        u[i] = base;
+        return;
+    }
+    const float delta = fminf(uf[i] - base, limit);
+    u[i] = base + delta;
+    const float2 t = tv[i];
+    float2 f = flow[i];
+    f.x = fmaf(delta, t.x, f.x);
+    f.y = fmaf(delta, t.y, f.y);
+    flow[i] = f;
+}
+
+__global__ void __launch_bounds__(256) k_add(float* __restrict__ out, const float* __restrict__ in, int64_t n)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) out[i] += in[i];
+}
+
+// One step of the fused reverse sweep over a 32 x 8 tile.  With S_i the state before
+// iteration i (S_0 the warp start), launch F(i) runs
+//   DO_D:   dual^T of iteration i (reads S_i: p, q, u_bar, v_bar) for the tile and a
+//           one-pixel ring, its u_bar / v_bar messages kept in LDS;
+//   DO_PL:  primal^T and threshold^T of iteration i - 1 (reads S_i: p, q and
+//           S_{i-1}: u, u_bar), collecting the messages from LDS;
+//   FINISH: instead of DO_PL, the adjoint of u_bar at the loop start (gub_out).
+// The head of a sweep is DO_PL alone (iteration n - 1, no messages yet), the tail
+// DO_D + FINISH (iteration 0).  Per-pixel arithmetic is that of the one-kernel-
+// per-step path; lambda / alpha shares are summed per block into double slots.
+struct FusedArgs {
+    float* gu;
+    float* gubB;
+    float2* gvbB;
+    const float2* gp;  // read (own pixel and, for the ring, the neighbouring tile's)
+    const float4* gq;
+    const float4* qmsg;
+    const float4* tmsg;
+    float2* gp_out;  // written (own pixel)
+    float4* gq_out;
+    float4* qmsg_out;
+    float4* tmsg_out;
+    float* gabc;
+    float* geta;
+    float2* gderiv;
+    double* gpart;
+    float* gub_out;
+    const float2* p1;  // S_i
+    const float4* q1;
+    const float* ub1;
+    const float2* vb1;
+    const float* u2;  // S_{i-1}
+    const float* ub2;
+    const float* abc;
+    const float* eta;
+    const float2* deriv;
+    const uint8_t* mask;
+    int H, W;
+    float sigma;          // iteration i
+    float tau_pl, mu_pl;  // iteration i - 1 (the head: n - 1)
+    float alpha0, alpha1, eta_p, eta_q, lambda;
+};
+
+constexpr int FL_X = FT_X + 2, FL_Y = FT_Y + 2, F_RING = 2 * FL_X + 2 * FT_Y;
+
+// dual^T at pixel (X, Y) (in the image).  gab: adjoint of abc on entry (the owner's
+// accumulator, or zeros for a ring pixel), updated.
+__device__ __forceinline__ void fused_dual(const FusedArgs& a, int X, int Y, float2& gpo, float4& gqo, DualMsg& msg,
+                                           float (&gab)[3], float& gpa0, float& gpa1)
+{
+    const int H = a.H, W = a.W, idx = Y * W + X;
+    const Nbr n = nbr(a.mask, X, Y, H, W);
+    float4 gqn = a.gq[idx];
+    float2 gtp;
+    collect_qt(a.qmsg, a.tmsg, n, X, Y, H, W, idx, gqn, gtp);
+    if (!n.m) {
+        gpo = make_float2(0.0f, 0.0f);
+        gqo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 3; ++k) msg.um[k] = 0.0f;
+        for (int k = 0; k < 6; ++k) msg.vm[k] = 0.0f;
+        return;
+    }
+    const bool fx = n.mE, fy = n.mS;
+    const int ix = fx ? n.iE : n.iW, iy = fy ? n.iS : n.iN;
+    dual_adj_pixel(fx, fy, a.ub1[idx], a.vb1[idx], a.ub1[ix], a.vb1[ix], a.ub1[iy], a.vb1[iy], a.abc[3 * idx],
+                   a.abc[3 * idx + 1], a.abc[3 * idx + 2], a.p1[idx], a.q1[idx], a.sigma, a.alpha0, a.alpha1, a.eta_p,
+                   a.eta_q, a.gp[idx], gqn, gtp, gpo, gqo, msg, gab, gpa0, gpa1);
+}
+
+struct LdsMsgs {
+    const float* m;  // FL_Y x FL_X x 9
+    int X0, Y0;
+    __device__ __forceinline__ DualMsg operator()(int x, int y) const
+    {
+        const float* r = m + 9 * ((y - Y0 + 1) * FL_X + (x - X0 + 1));
+        DualMsg d;
+        for (int k = 0; k < 3; ++k) d.um[k] = r[k];
+        for (int k = 0; k < 6; ++k) d.vm[k] = r[3 + k];
+        return d;
+    }
+};
+
+__device__ __forceinline__ void put_msg(float* m, int l, const DualMsg& d)
+{
+    float* r = m + 9 * l;
+    for (int k = 0; k < 3; ++k) r[k] = d.um[k];
+    for (int k = 0; k < 6; ++k) r[3 + k] = d.vm[k];
+}
+
+template <bool DO_D, bool DO_PL, bool FINISH>
+__global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
+{
+    __shared__ float s_msg[FL_Y * FL_X * 9];
+    __shared__ float s_red[3 * 4];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FT_X + tx;
+    const int X0 = blockIdx.x * FT_X, Y0 = blockIdx.y * FT_Y;
+    const int X = X0 + tx, Y = Y0 + ty;
+    const int H = a.H, W = a.W;
+    const bool inside = X < W && Y < H;
+    const int idx = inside ? Y * W + X : 0;
+    float gpl = 0.0f, gpa0 = 0.0f, gpa1 = 0.0f;
+    float2 gpo = make_float2(0.0f, 0.0f);
+    float4 gqo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float gab[3] = {0.0f, 0.0f, 0.0f};
+    if (inside) {
+        gab[0] = a.gabc[3 * idx];
+        gab[1] = a.gabc[3 * idx + 1];
+        gab[2] = a.gabc[3 * idx + 2];
+    }
+    if constexpr (DO_D) {
+        if (inside) {
+            DualMsg m;
+            fused_dual(a, X, Y, gpo, gqo, m, gab, gpa0, gpa1);
+            put_msg(s_msg, (ty + 1) * FL_X + tx + 1, m);
+        }
+        if (tid < F_RING) {
+            int lx, ly;
+            if (tid < FL_X) {
+                lx = tid;
+                ly = 0;
+            } else if (tid < 2 * FL_X) {
+                lx = tid - FL_X;
+                ly = FL_Y - 1;
+            } else if (tid < 2 * FL_X + FT_Y) {
+                lx = 0;
+                ly = 1 + tid - 2 * FL_X;
+            } else {
+                lx = FL_X - 1;
+                ly = 1 + tid - 2 * FL_X - FT_Y;
+            }
+            const int gx = X0 - 1 + lx, gy = Y0 - 1 + ly;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                float2 p_;
+                float4 q_;
+                float g_[3] = {0.0f, 0.0f, 0.0f};
+                float d0 = 0.0f, d1 = 0.0f;
+                DualMsg m;
+                fused_dual(a, gx, gy, p_, q_, m, g_, d0, d1);
+                put_msg(s_msg, ly * FL_X + lx, m);
+            }
+        }
+        __syncthreads();
+    }
+    if (inside) {
+        if constexpr (DO_PL) {
+            float g_ubo = a.gubB[idx];
+            float2 g_vbo = a.gvbB[idx];
+            if constexpr (DO_D) gather_msgs(a.mask, LdsMsgs{s_msg, X0, Y0}, X, Y, H, W, g_ubo, g_vbo);
+            const Nbr n = nbr(a.mask, X, Y, H, W);
+            if (!n.m) {
+                a.gubB[idx] = 0.0f;
+                a.gvbB[idx] = make_float2(0.0f, 0.0f);
+                a.qmsg_out[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                a.gu[idx] = 0.0f;
+                a.tmsg_out[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                a.gp_out[idx] = DO_D ? gpo : a.gp[idx];
+                a.gq_out[idx] = DO_D ? gqo : a.gq[idx];
+            } else {
+                const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                float2 gp_ = DO_D ? gpo : a.gp[idx];
+                float4 gq_ = DO_D ? gqo : a.gq[idx];
+                float gu_ = a.gu[idx];
+                float e0 = a.geta[3 * idx], e1 = a.geta[3 * idx + 1], e2 = a.geta[3 * idx + 2];
+                float ubB;
+                float2 vbB;
+                float4 qm, tm;
+                const float ea = a.abc[3 * idx], eb = a.abc[3 * idx + 1], ec = a.abc[3 * idx + 2];
+                primal_adj_pixel(n, a.q1[idx], n.mW ? a.q1[n.iW] : zero4, n.mN ? a.q1[n.iN] : zero4, ea, eb, ec,
+                                 a.p1[idx], a.eta[3 * idx + 1], a.eta[3 * idx + 2], a.tau_pl, a.mu_pl, a.alpha0,
+                                 a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_, qm, gab, e1, e2, gpa0, gpa1);
+                float2 gd = a.gderiv[idx];
+                gu_ = l1_adj_pixel(n, gu_, tp_at(a.p1, a.abc, a.mask, idx), tp_at(a.p1, a.abc, a.mask, n.iW).x,
+                                   tp_at(a.p1, a.abc, a.mask, n.iN).y, a.deriv[idx], a.eta[3 * idx], a.u2[idx],
+                                   a.ub2[idx], a.tau_pl, a.lambda, tm, ubB, e0, gd, gpl);
+                a.gu[idx] = gu_;
+                a.gubB[idx] = ubB;
+                a.gvbB[idx] = vbB;
+                a.gp_out[idx] = gp_;
+                a.gq_out[idx] = gq_;
+                a.qmsg_out[idx] = qm;
+                a.tmsg_out[idx] = tm;
+                a.geta[3 * idx] = e0;
+                a.geta[3 * idx + 1] = e1;
+                a.geta[3 * idx + 2] = e2;
+                a.gderiv[idx] = gd;
+                for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[k];
+            }
+        } else {
+            // DO_D alone: the dual^T outputs of the owned pixel
+            a.gp_out[idx] = gpo;
+            a.gq_out[idx] = gqo;
+            for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[k];
+            if constexpr (FINISH) {
+                float g = a.gubB[idx];
+                float2 gv = a.gvbB[idx];
+                gather_msgs(a.mask, LdsMsgs{s_msg, X0, Y0}, X, Y, H, W, g, gv);
+                a.gub_out[idx] = g;
+            }
+        }
+    }
+    // block sums of the lambda / alpha shares (wavefront shuffles, then four wavefronts)
+    float v[3] = {gpl, gpa0, gpa1};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if ((tid & 63) == 0)
+        for (int k = 0; k < 3; ++k) s_red[3 * (tid >> 6) + k] = v[k];
+    __syncthreads();
+    if (tid == 0) {
+        double* slot = a.gpart + 3 * (static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x);
+        for (int k = 0; k < 3; ++k)
+            slot[k] += static_cast<double>(s_red[k]) + static_cast<double>(s_red[3 + k]) +
+                       static_cast<double>(s_red[6 + k]) + static_cast<double>(s_red[9 + k]);
+    }
+}
+
+// Adds each fused tile's (lambda, alpha0, alpha1) sums to the per-pixel shares of
+// the tile's first pixel.
+__global__ void __launch_bounds__(256) k_fold_gpart(float* __restrict__ gpar, const double* __restrict__ gpart,
+                                                    int H, int W, int nbx, int nblocks)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int X = (b % nbx) * FT_X, Y = (b / nbx) * FT_Y;
+    float* g = gpar + static_cast<int64_t>(GP_N) * (static_cast<int64_t>(Y) * W + X);
+    g[GP_LAMBDA] += static_cast<float>(gpart[3 * b]);
+    g[GP_ALPHA0] += static_cast<float>(gpart[3 * b + 1]);
+    g[GP_ALPHA1] += static_cast<float>(gpart[3 * b + 2]);
+}
+
+inline unsigned nb1(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+#define VFS_TRY(expr)             \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+int check_args(const vfs_level* lv, const vfs_params* prm, const float* taus, const float* sigmas, const float* mus,
+               int n_iter, int n_warp, int ck)
+{
+    if (lv == nullptr || prm == nullptr || n_iter < 0 || n_warp < 0 || ck < 1) return VFS_EINVAL;
+    VFS_CHECK_DIMS(lv->H, lv->W);
+    if (n_iter > 0 && (taus == nullptr || sigmas == nullptr || mus == nullptr)) return VFS_EINVAL;
+    if (prm->median_k != 3 && prm->median_k != 5) return VFS_EINVAL;
+    const void* ptrs[] = {lv->ref, lv->other, lv->tvf, lv->mask, lv->flow, lv->u, lv->u_bar, lv->ref_smoothed,
+                          lv->abc, lv->eta, lv->tv, lv->warped, lv->deriv, lv->p, lv->q, lv->tp, lv->v_bar,
+                          lv->u0, lv->uf};
+    for (const void* p : ptrs)
+        if (p == nullptr) return VFS_EINVAL;
+    return 0;
+}
+
+// State S_{i+1} from S_i (iteration i of the schedule), written out of place.
+int pd_step(void* st, const vfs_level* lv, const vfs_params* prm, const State& a, const State& b, float* tp, float tau,
+            float sigma, float mu)
+{
+    const int H = lv->H, W = lv->W;
+    hipLaunchKernelGGL(k_dual_oop, grid2d(H, W), block2d(), 0, S(st), reinterpret_cast<float2*>(b.p),
+                       reinterpret_cast<float2*>(tp), reinterpret_cast<float4*>(b.q),
+                       reinterpret_cast<const float2*>(a.p), reinterpret_cast<const float4*>(a.q), sigma, a.ub,
+                       reinterpret_cast<const float2*>(a.vb), lv->abc, lv->mask, H, W, prm->alpha0, prm->alpha1,
+                       prm->eta_p, prm->eta_q);
+    if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+    VFS_TRY(vfs_l1_thresholding(st, b.u, tau, tp, a.u, a.ub, lv->deriv, lv->eta, lv->mask, H, W, prm));
+    VFS_TRY(vfs_update_primal(st, b.ub, b.vb, tau, mu, b.p, b.q, b.u, a.ub, a.vb, lv->abc, lv->eta, lv->mask, H, W,
+                              prm));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t vfs_adj_save_bytes(int H, int W, int n_warp, int n_iter, int ckpt)
+{
+    if (H <= 0 || W <= 0 || n_warp < 0 || n_iter < 0 || ckpt < 1) return 0;
+    const int64_t n = static_cast<int64_t>(H) * W;
+    return static_cast<size_t>(n_warp) * static_cast<size_t>(warp_save_floats(n, n_iter, ckpt)) * sizeof(float);
+}
+
+size_t vfs_adj_workspace_bytes(int H, int W, int ckpt)
+{
+    if (H <= 0 || W <= 0 || ckpt < 1) return 0;
+    const int64_t n = static_cast<int64_t>(H) * W;
+    return static_cast<size_t>(adj_ws_floats(n, fused_blocks(H, W)) + static_cast<int64_t>(ckpt) * state_floats(n)) *
+           sizeof(float);
+}
+
+int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
+                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save)
+{
+    VFS_TRY(check_args(lv, prm, taus, sigmas, mus, n_iter, n_warp, ckpt));
+    if (save == nullptr && n_warp > 0) return VFS_EINVAL;
+    const int H = lv->H, W = lv->W;
+    const int64_t n = static_cast<int64_t>(H) * W;
+    hipStream_t s = S(stream);
+    VFS_TRY(vfs_gaussian_masked(stream, lv->ref_smoothed, lv->ref, lv->mask, H, W));
+    VFS_TRY(vfs_compute_tensor_eta(stream, lv->abc, lv->eta, lv->ref_smoothed, lv->mask, H, W, prm));
+    for (int w = 0; w < n_warp; ++w) {
+        const WarpSave ws = warp_save_at(save, n, n_iter, ckpt, w);
+        VFS_TRY(vfs_find_warping_vector(stream, lv->tv, lv->flow, H, W, lv->tvf, H, W));
+        VFS_TRY(vfs_warp_image(stream, lv->warped, lv->flow, H, W, lv->other, H, W));
+        VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+        hipLaunchKernelGGL(k_warp_reset_save, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(lv->p),
+                           reinterpret_cast<float4*>(lv->q), reinterpret_cast<float2*>(lv->v_bar), lv->u0, lv->u,
+                           lv->u_bar, reinterpret_cast<const float2*>(lv->flow), ws.u, ws.ub,
+                           reinterpret_cast<float2*>(ws.flow), static_cast<int>(n));
+        if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        for (int it = 0; it < n_iter; ++it) {
+            if (it % ckpt == 0) {
+                const State c = state_at(ws.ckpt0 + static_cast<int64_t>(it / ckpt) * state_floats(n), n);
+                hipLaunchKernelGGL(k_save_state, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(c.p),
+                                   reinterpret_cast<float4*>(c.q), c.u, c.ub, reinterpret_cast<float2*>(c.vb),
+                                   reinterpret_cast<const float2*>(lv->p), reinterpret_cast<const float4*>(lv->q),
+                                   lv->u, lv->u_bar, reinterpret_cast<const float2*>(lv->v_bar), static_cast<int>(n));
+                if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            }
+            VFS_TRY(vfs_update_dual(stream, lv->p, lv->tp, lv->q, sigmas[it], lv->u_bar, lv->v_bar, lv->abc, lv->mask,
+                                    H, W, prm));
+            VFS_TRY(vfs_l1_thresholding(stream, lv->u, taus[it], lv->tp, lv->u, lv->u_bar, lv->deriv, lv->eta,
+                                        lv->mask, H, W, prm));
+            VFS_TRY(vfs_update_primal(stream, lv->u_bar, lv->v_bar, taus[it], mus[it], lv->p, lv->q, lv->u,
+                                      lv->u_bar, lv->v_bar, lv->abc, lv->eta, lv->mask, H, W, prm));
+        }
+        VFS_TRY(vfs_median_filter(stream, lv->uf, lv->u, lv->mask, H, W, prm->median_k));
+        hipLaunchKernelGGL(k_flow_clamp_u, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(lv->flow), lv->u,
+                           lv->u0, lv->uf, reinterpret_cast<const float2*>(lv->tv), lv->mask, static_cast<int>(n),
+                           prm->limit_range);
+        if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+    }
+    return 0;
+}
+
+int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
+                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, const float* save,
+                          float* workspace, float* gu, float* gub, float* gf, float* gref, float* gpar, int mode)
+{
+    VFS_TRY(check_args(lv, prm, taus, sigmas, mus, n_iter, n_warp, ckpt));
+    const void* ptrs[] = {workspace, gu, gub, gf, gref, gpar};
+    for (const void* p : ptrs)
+        if (p == nullptr) return VFS_EINVAL;
+    if (save == nullptr && n_warp > 0) return VFS_EINVAL;
+    if (mode != 0 && mode != 1) return VFS_EINVAL;
+    const int H = lv->H, W = lv->W;
+    const int64_t n = static_cast<int64_t>(H) * W;
+    hipStream_t s = S(stream);
+    float* sv = const_cast<float*>(save);
+    const int nbx = (W + FT_X - 1) / FT_X, nby = (H + FT_Y - 1) / FT_Y;
+    const int64_t nblocks = static_cast<int64_t>(nbx) * nby;
+    const AdjWs A = adj_ws_at(workspace, n, nblocks);
+    const int nck = n_ckpt(n_iter, ckpt);
+    VFS_TRY(vfs_fill(stream, A.gabc, 0.0f, 3 * n));
+    VFS_TRY(vfs_fill(stream, A.geta, 0.0f, 3 * n));
+    VFS_TRY(vfs_fill(stream, reinterpret_cast<float*>(A.gpart), 0.0f, 2 * 3 * nblocks));
+    FusedArgs fa;
+    fa.gu = A.guth;
+    fa.gubB = A.gubB;
+    fa.gvbB = reinterpret_cast<float2*>(A.gvbB);
+    fa.gabc = A.gabc;
+    fa.geta = A.geta;
+    fa.gderiv = reinterpret_cast<float2*>(A.gderiv);
+    fa.gpart = A.gpart;
+    fa.gub_out = gub;
+    fa.abc = lv->abc;
+    fa.eta = lv->eta;
+    fa.deriv = reinterpret_cast<const float2*>(lv->deriv);
+    fa.mask = lv->mask;
+    fa.H = H;
+    fa.W = W;
+    fa.alpha0 = prm->alpha0;
+    fa.alpha1 = prm->alpha1;
+    fa.eta_p = prm->eta_p;
+    fa.eta_q = prm->eta_q;
+    fa.lambda = prm->lambda;
+    const dim3 fgrid(nbx, nby), fblock(FT_X, FT_Y);
+    for (int w = n_warp - 1; w >= 0; --w) {
+        const WarpSave ws = warp_save_at(sv, n, n_iter, ckpt, w);
+        VFS_TRY(vfs_find_warping_vector(stream, lv->tv, ws.flow, H, W, lv->tvf, H, W));
+        VFS_TRY(vfs_warp_image(stream, lv->warped, ws.flow, H, W, lv->other, H, W));
+        VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+        // state i of segment seg = [a, b]: the checkpoint for i == a, else tape slot i - a - 1
+        auto state_of = [&](int seg, int i) {
+            const int a = seg * ckpt;
+            if (i == a) return state_at(ws.ckpt0 + static_cast<int64_t>(seg) * state_floats(n), n);
+            return state_at(A.tape + static_cast<int64_t>(i - a - 1) * state_floats(n), n);
+        };
+        auto recompute = [&](int seg) {
+            const int a = seg * ckpt, b = (seg + 1) * ckpt < n_iter ? (seg + 1) * ckpt : n_iter;
+            for (int i = a; i < b; ++i)
+                VFS_TRY(pd_step(stream, lv, prm, state_of(seg, i), state_of(seg, i + 1), A.tp, taus[i], sigmas[i],
+                                mus[i]));
+            return 0;
+        };
+        const float* u_th = ws.u;
+        if (nck > 0) {
+            VFS_TRY(recompute(nck - 1));
+            u_th = state_of(nck - 1, n_iter).u;
+        }
+        VFS_TRY(vfs_adj_clamp(stream, gu, A.guf, A.pos, A.gtv, gf, u_th, ws.u, lv->tv, lv->mask, H, W,
+                              prm->limit_range, prm->median_k));
+        VFS_TRY(vfs_adj_median(stream, A.guth, A.guf, A.pos, H, W, prm->median_k));
+        VFS_TRY(vfs_copy(stream, A.gubB, gub, n));
+        VFS_TRY(vfs_fill(stream, A.gvbB, 0.0f, 2 * n));
+        VFS_TRY(vfs_fill(stream, A.gp, 0.0f, 2 * n));
+        VFS_TRY(vfs_fill(stream, A.gq, 0.0f, 4 * n));
+        VFS_TRY(vfs_fill(stream, A.gderiv, 0.0f, 2 * n));
+        if (nck > 0 && mode == 0) {
+            // fused sweep: head PL(n-1), F(i) for i = n-1 .. 1, tail D(0) + finish
+            int cs = nck - 1;
+            // ping-pong of (gp, gq, qmsg, tmsg): set 0 is A.gp ..., set 1 the second copies;
+            // the head reads set 0 (gp, gq cleared above)
+            int cur = 0;
+            auto set_bufs = [&]() {
+                float* gpi[2] = {A.gp, A.gp2};
+                float* gqi[2] = {A.gq, A.gq2};
+                float* qmi[2] = {A.qmsg, A.qmsg2};
+                float* tmi[2] = {A.tmsg, A.tmsg2};
+                fa.gp = reinterpret_cast<const float2*>(gpi[cur]);
+                fa.gq = reinterpret_cast<const float4*>(gqi[cur]);
+                fa.qmsg = reinterpret_cast<const float4*>(qmi[cur]);
+                fa.tmsg = reinterpret_cast<const float4*>(tmi[cur]);
+                fa.gp_out = reinterpret_cast<float2*>(gpi[1 - cur]);
+                fa.gq_out = reinterpret_cast<float4*>(gqi[1 - cur]);
+                fa.qmsg_out = reinterpret_cast<float4*>(qmi[1 - cur]);
+                fa.tmsg_out = reinterpret_cast<float4*>(tmi[1 - cur]);
+                cur = 1 - cur;
+            };
+            auto set_states = [&](const State& si, const State& sim1) {
+                fa.p1 = reinterpret_cast<const float2*>(si.p);
+                fa.q1 = reinterpret_cast<const float4*>(si.q);
+                fa.ub1 = si.ub;
+                fa.vb1 = reinterpret_cast<const float2*>(si.vb);
+                fa.u2 = sim1.u;
+                fa.ub2 = sim1.ub;
+            };
+            set_states(state_of(cs, n_iter), state_of(cs, n_iter - 1));
+            fa.sigma = 0.0f;
+            fa.tau_pl = taus[n_iter - 1];
+            fa.mu_pl = mus[n_iter - 1];
+            set_bufs();
+            hipLaunchKernelGGL((k_adj_fused<false, true, false>), fgrid, fblock, 0, s, fa);
+            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            for (int i = n_iter - 1; i >= 1; --i) {
+                if (i - 1 < cs * ckpt) {
+                    --cs;
+                    VFS_TRY(recompute(cs));
+                }
+                set_states(state_of(cs, i), state_of(cs, i - 1));
+                fa.sigma = sigmas[i];
+                fa.tau_pl = taus[i - 1];
+                fa.mu_pl = mus[i - 1];
+                set_bufs();
+                hipLaunchKernelGGL((k_adj_fused<true, true, false>), fgrid, fblock, 0, s, fa);
+                if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            }
+            const State s0 = state_at(ws.ckpt0, n);
+            set_states(s0, s0);
+            fa.sigma = sigmas[0];
+            set_bufs();
+            hipLaunchKernelGGL((k_adj_fused<true, false, true>), fgrid, fblock, 0, s, fa);
+            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        } else if (nck > 0) {
+            // one kernel per step (primal^T, threshold^T, dual^T), messages through memory
+            int has_msgs = 0;
+            for (int seg = nck - 1; seg >= 0; --seg) {
+                if (seg != nck - 1) VFS_TRY(recompute(seg));
+                const int a = seg * ckpt, b = (seg + 1) * ckpt < n_iter ? (seg + 1) * ckpt : n_iter;
+                for (int it = b - 1; it >= a; --it) {
+                    const State after = state_of(seg, it + 1), before = state_of(seg, it);
+                    vfs_adj_pd ad;
+                    ad.H = H;
+                    ad.W = W;
+                    ad.gu = A.guth;
+                    ad.gub = A.gubB;
+                    ad.gvb = A.gvbB;
+                    ad.gp = A.gp;
+                    ad.gq = A.gq;
+                    ad.um = A.um;
+                    ad.vm = A.vm;
+                    ad.qmsg = A.qmsg;
+                    ad.tmsg = A.tmsg;
+                    ad.gabc = A.gabc;
+                    ad.geta = A.geta;
+                    ad.gderiv = A.gderiv;
+                    ad.gpar = gpar;
+                    ad.p = after.p;
+                    ad.q = after.q;
+                    ad.u = after.u;
+                    ad.p_prev = before.p;
+                    ad.q_prev = before.q;
+                    ad.u_prev = before.u;
+                    ad.u_bar_prev = before.ub;
+                    ad.v_bar_prev = before.vb;
+                    ad.abc = lv->abc;
+                    ad.eta = lv->eta;
+                    ad.deriv = lv->deriv;
+                    ad.mask = lv->mask;
+                    VFS_TRY(vfs_adj_pd_iteration(stream, &ad, prm, taus[it], sigmas[it], mus[it], has_msgs));
+                    has_msgs = 1;
+                }
+            }
+            // adjoint of u_bar at the warp start; v_bar, p, q start at zero
+            VFS_TRY(vfs_adj_pd_finish(stream, gub, A.gvb_fin, A.gubB, A.gvbB, A.um, A.vm, lv->mask, H, W));
+        }
+        hipLaunchKernelGGL(k_add, dim3(nb1(n)), dim3(256), 0, s, gu, A.guth, n);
+        if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        VFS_TRY(vfs_fill(stream, A.gow, 0.0f, n));
+        VFS_TRY(vfs_adj_derivatives(stream, gref, A.gow, A.gderiv, lv->tvf, H, W));
+        VFS_TRY(vfs_adj_warp(stream, gf, A.gow, A.gtv, ws.flow, lv->other, lv->tvf, H, W));
+    }
+    if (n_warp > 0 && nck > 0 && mode == 0) {
+        hipLaunchKernelGGL(k_fold_gpart, dim3(nb1(nblocks)), dim3(256), 0, s, gpar, A.gpart, H, W, nbx,
+                           static_cast<int>(nblocks));
+        if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+    }
+    VFS_TRY(vfs_fill(stream, A.gIs, 0.0f, n));
+    VFS_TRY(vfs_adj_tensor(stream, A.gIs, gpar, A.gabc, A.geta, lv->ref_smoothed, lv->mask, H, W, prm));
+    VFS_TRY(vfs_adj_gaussian(stream, gref, A.gIs, lv->mask, H, W));
+    return 0;
+}
+
+}  // extern "C"

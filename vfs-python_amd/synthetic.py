@@ -279,6 +279,9 @@ WORKLOADS = {
                             nLevel=3, nWarpIters=10, nSolverIters=20, **_SOLVER),
     "plain_full_800": dict(H=800, W=800, t=(0.3, 0.0, 0.0), rot_deg=2.0, rot_random=True,
                            nLevel=5, nWarpIters=20, nSolverIters=50, **_SOLVER),
+    # gradient of the depth loss: 3 levels x 10 warps x 50 iterations, checkpoint every 25
+    "adjoint_800": dict(H=800, W=800, t=(0.3, 0.0, 0.0), rot_deg=2.0, rot_random=True,
+                        nLevel=3, nWarpIters=10, nSolverIters=50, checkpoint=25, pairs=8, **_SOLVER),
 }
 
 # 64 pairs, 16 each of four sizes, every pair with its own pose and scene.

@@ -110,6 +110,10 @@ _SIGS = {
     "vfs_adj_upsample": [c_p, c_p, c_i, c_i, c_p, c_i, c_i, c_i, c_f],
     "vfs_depth_loss": [c_p] * 6 + [c_i, c_i, c_f, c_f, c_f, c_f, c_f],
     "vfs_reduce_sum": [c_p, c_p, c_p, c_i, c_p, c_i64, c_i],
+    "vfs_adj_forward_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
+                              c_p],
+    "vfs_adj_reverse_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
+                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i],
     "vfs_capture_begin": [ctypes.POINTER(c_p)],
     "vfs_capture_end": [c_p, ctypes.POINTER(c_p)],
     "vfs_plan_launch": [c_p, c_p],
@@ -117,9 +121,17 @@ _SIGS = {
 }
 
 
+# entry points returning size_t
+_SIZE_SIGS = {
+    "vfs_batch_workspace_bytes": [c_i],
+    "vfs_adj_save_bytes": [c_i, c_i, c_i, c_i, c_i],
+    "vfs_adj_workspace_bytes": [c_i, c_i, c_i],
+}
+
+
 def exported_symbols():
     """Names of the C ABI entry points this binding expects (include/vfs_hip.h)."""
-    return sorted(list(_SIGS) + ["vfs_version", "vfs_batch_workspace_bytes"])
+    return sorted(list(_SIGS) + list(_SIZE_SIGS) + ["vfs_version"])
 
 
 def lib():
@@ -135,8 +147,10 @@ def lib():
             fn.restype = c_i
         L.vfs_version.restype = ctypes.c_char_p
         L.vfs_version.argtypes = []
-        L.vfs_batch_workspace_bytes.restype = ctypes.c_size_t
-        L.vfs_batch_workspace_bytes.argtypes = [c_i]
+        for name, args in _SIZE_SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_size_t
         _lib = L
     return _lib
 
