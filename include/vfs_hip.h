@@ -392,26 +392,43 @@ int vfs_reduce_sum(void* stream, double* out, double* partial, int nparts, const
 size_t vfs_adj_save_bytes(int H, int W, int n_warp, int n_iter, int ckpt);
 size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
 
-/* Same results as vfs_solve_level_plain (the per-kernel warp loop of
- * Python/vfs.py:473-521), plus the save area. */
-int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
-                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save);
+/* Kernel choices of the adjoint level driver.  A zero variant selects the
+ * per-kernel path.
+ *   fwd_tile_x, fwd_tile_y, fwd_k: blocked primal-dual variant (vfs_pd_variants) of
+ *     the forward; its launches end at every checkpoint;
+ *   rec_tile_x, rec_tile_y, rec_k: blocked variant of the tape recompute; each launch
+ *     runs up to rec_k iterations and writes every intermediate state to the tape;
+ *   sweep: reverse primal-dual sweep.  0: one fused kernel per iteration (dual^T of
+ *     iteration i with primal^T and threshold^T of iteration i - 1; the dual
+ *     messages of a 32 x 8 tile and its one-pixel ring exchanged in LDS); 1: one
+ *     kernel per step (vfs_adj_pd_iteration), messages through memory; 11, 12, 14:
+ *     fused with tiles of 32 x 8, 32 x 16, 32 x 32 (1, 2, 4 rows per thread). */
+typedef struct vfs_adj_tiles {
+    int32_t fwd_tile_x, fwd_tile_y, fwd_k;
+    int32_t rec_tile_x, rec_tile_y, rec_k;
+    int32_t sweep;
+} vfs_adj_tiles;
 
-/* Reverse of vfs_adj_forward_level.  lv must still hold the forward's ref_smoothed,
+/* Same results as vfs_solve_level_plain (the per-kernel warp loop of
+ * Python/vfs.py:473-521; the blocked variants give the same values), plus the
+ * save area.  workspace (vfs_adj_workspace_bytes) holds the second state set of
+ * the blocked path (unused by the per-kernel path); tiles may be NULL (per-kernel). */
+int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
+                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save,
+                          float* workspace, const vfs_adj_tiles* tiles);
+
+/* Reverse of vfs_adj_forward_level (tiles: see vfs_adj_tiles; NULL = per-kernel
+ * recompute and sweep 0).  lv must still hold the forward's ref_smoothed,
  * abc and eta; its tv, warped, deriv, p, q, tp, v_bar, u0, uf are overwritten.
  * In: gu, gub (H,W) and gf (H,W,2) are the adjoints of the level's final u, u_bar
  * and flow; on return they are the adjoints of the level's initial u, u_bar and
  * flow.  gref (H,W) += adjoint of the level's reference image; gpar (H,W,5) +=
  * shares of lambda, alpha0, alpha1, beta, gamma whose sum over all pixels is the
- * level's contribution to those gradients.
- * mode 0: the primal-dual reverse runs as one fused kernel per iteration
- * (dual^T of iteration i with primal^T and threshold^T of iteration i - 1, the
- * dual messages of a 32 x 8 tile and its one-pixel ring exchanged in LDS);
- * mode 1: one kernel per step (vfs_adj_pd_iteration), messages through memory. */
+ * level's contribution to those gradients. */
 int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
                           const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt,
                           const float* save, float* workspace, float* gu, float* gub, float* gf, float* gref,
-                          float* gpar, int mode);
+                          float* gpar, const vfs_adj_tiles* tiles);
 
 /* Library version string (static storage). */
 const char* vfs_version(void);

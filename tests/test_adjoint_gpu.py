@@ -141,14 +141,14 @@ def _small_case(n=48, seed=5, levels=2):
     return pr, S.sanitize_field(tvf), cvf.astype(np.float32)
 
 
-def _adjoint(pr, tvf, cvf, n_warp, n_iter, ckpt, levels=2):
+def _adjoint(pr, tvf, cvf, n_warp, n_iter, ckpt, levels=2, **kw):
     import vfs_adjoint as A
     import vfs_params as P
     prm = P.vfs_params(pr["img_ref"])
     prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = levels, 2.0, n_warp, n_iter
     for k, v in THETA.items():
         setattr(prm, k, v)
-    return A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=ckpt)
+    return A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=ckpt, **kw)
 
 
 def test_solve_adjoint_matches_autograd_through_unrolled_torch(T):
@@ -226,12 +226,14 @@ def test_solve_adjoint_matches_central_differences_of_the_oracle(T):
         assert abs(g[k] - fd) <= tol, k
 
 
-def test_forward_level_driver_matches_plain_solver(T):
-    """vfs_adj_forward_level (plain warp loop + saves) gives the plain solver's flow bit for bit."""
+@pytest.mark.parametrize("blocked", [False, True])
+def test_forward_level_driver_matches_plain_solver(T, blocked):
+    """vfs_adj_forward_level (per-kernel or blocked warp loop + saves) gives the plain solver's
+    flow bit for bit."""
     import vfs as V
     import vfs_params as P
     pr, tvf, cvf = _small_case(64, 3)
-    adj = _adjoint(pr, tvf, cvf, 3, 7, ckpt=3)
+    adj = _adjoint(pr, tvf, cvf, 3, 7, ckpt=3, blocked=blocked)
     ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
     adj.forward(ref0, pr["img_other"])
     prm = P.vfs_params(pr["img_ref"])
@@ -249,12 +251,15 @@ def test_forward_level_driver_matches_plain_solver(T):
     assert torch.equal(flow, adj.flow)
 
 
-def test_gradients_do_not_depend_on_the_checkpoint_interval(T):
+def test_gradients_do_not_depend_on_checkpoints_or_kernel_choice(T):
+    """Checkpoint interval, per-kernel or blocked forward, per-kernel or blocked (k = 1)
+    tape recompute: the states are the same bits, so the gradients agree."""
     pr, tvf, cvf = _small_case(48, 4)
     ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
     res = []
-    for ck in (1, 4, 9):
-        adj = _adjoint(pr, tvf, cvf, 2, 9, ckpt=ck)
+    for ck, blocked, rec in ((1, False, None), (4, True, (16, 16, 1)), (9, True, (32, 32, 4)),
+                             (4, False, (16, 16, 2)), (9, True, None), (9, True, (16, 16, 8)), (5, True, "auto")):
+        adj = _adjoint(pr, tvf, cvf, 2, 9, ckpt=ck, blocked=blocked, rec_tile=rec)
         loss = adj.forward(ref0, pr["img_other"])
         res.append((loss, adj.backward()))
     l0, g0 = res[0]
@@ -266,8 +271,9 @@ def test_gradients_do_not_depend_on_the_checkpoint_interval(T):
         assert float((g["ref0"] - g0["ref0"]).abs().max()) <= 1e-5 * float(g0["ref0"].abs().max())
 
 
-@pytest.mark.parametrize("shape,ckpt", [((54, 70), 4), ((48, 48), 9)])
-def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt):
+@pytest.mark.parametrize("shape,ckpt,rows", [((54, 70), 4, None), ((48, 48), 9, None), ((54, 70), 4, 1),
+                                             ((54, 70), 5, 2), ((54, 70), 3, 4), ((100, 68), 4, 4)])
+def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt, rows):
     """The fused reverse sweep (dual^T + primal^T + threshold^T per launch, ring messages in
     LDS) against the one-kernel-per-step sweep; partial tiles at the right / bottom edges."""
     import synthetic as S
@@ -285,7 +291,7 @@ def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt):
     res = []
     for fused in (False, True):
         adj = A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=ckpt,
-                                fused=fused)
+                                fused=fused, rows=rows)
         loss = adj.forward(ref0, pr["img_other"])
         res.append((loss, adj.backward()))
     (l0, g0), (l1, g1) = res

@@ -104,7 +104,7 @@ __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4
                                                  float alpha0, float alpha1, float g_ubo, float2 g_vbo,
                                                  float& gubB, float2& gvbB, float& gu, float2& gp, float4& gq,
                                                  float4& qmsg, float (&gabc)[3], float& geta1, float& geta2,
-                                                 float& gpa0, float& gpa1)
+                                                 float& gpa0, float& gpa1, float2& qmsg_c)
 {
     float4 gQ = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (n.mE) {
@@ -156,8 +156,11 @@ __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4
         gq.z += g_S1;
         gq.y += g_S2;
     }
-    // messages: to W (x, w components of q_W), to N (y, z components of q_N)
+    // messages: to W (x, w components of q_W), to N (y, z components of q_N).  The W / N
+    // gating is the receiver's own mask, so qmsg_c = (-g_S1, -g_S2) carries the same
+    // information for a receiver that knows it is inside the mask.
     qmsg = make_float4(n.mW ? -g_S1 : 0.0f, n.mW ? -g_S2 : 0.0f, n.mN ? -g_S2 : 0.0f, n.mN ? -g_S1 : 0.0f);
+    qmsg_c = make_float2(-g_S1, -g_S2);
 }
 
 // T p of a pixel (zero where masked): the tp the forward threshold read.
@@ -177,7 +180,7 @@ __device__ __forceinline__ float2 tp_at(const float2* __restrict__ p, const floa
 // adds to gubB, geta.x, gderiv and the lambda share.
 __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 tc, float tpWx, float tpNy, float2 d,
                                               float e0, float uc, float ubc, float tau, float lambda, float4& tmsg,
-                                              float& gubB, float& geta0, float2& gderiv, float& gplam)
+                                              float& gubB, float& geta0, float2& gderiv, float& gplam, float& g_g_out)
 {
     float cxc, cxw, cyc, cyn;
     if (n.mE && n.mW) {
@@ -239,7 +242,10 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
     gderiv.x += g_dIu;
     gderiv.y += g_dIz;
     gplam += g_lam;
+    // cxc = [E inside], cyc = [S inside]; for a receiver inside the mask cxw = cyn = -1,
+    // so g_g alone carries the messages
     tmsg = make_float4(cxc * g_g, cyc * g_g, cxw * g_g, cyn * g_g);
+    g_g_out = g_g;
     return gout - g_d0;
 }
 
@@ -275,6 +281,39 @@ __device__ __forceinline__ void collect_qt(const float4* __restrict__ qmsg, cons
     if (X + 1 < W) gtp.x += tmsg[idx + 1].z;
     if (n.iN == idx) gtp.y += t0.w;
     if (Y + 1 < H) gtp.y += tmsg[idx + W].w;
+}
+
+// collect_qt for the compact messages qm = (-g_S1, -g_S2) and tm = g_g of the senders;
+// valid for a receiver inside the mask (a masked receiver discards the result).
+__device__ __forceinline__ void collect_qt_compact(const float2* __restrict__ qm, const float* __restrict__ tm,
+                                                   const Nbr& n, int X, int Y, int H, int W, int idx, float4& gqn,
+                                                   float2& gtp)
+{
+    const float2 q0 = qm[idx];
+    if (X == 0) {
+        gqn.x += q0.x;
+        gqn.w += q0.y;
+    }
+    if (X + 1 < W) {
+        const float2 me = qm[idx + 1];
+        gqn.x += me.x;
+        gqn.w += me.y;
+    }
+    if (Y == 0) {
+        gqn.y += q0.y;
+        gqn.z += q0.x;
+    }
+    if (Y + 1 < H) {
+        const float2 ms = qm[idx + W];
+        gqn.y += ms.y;
+        gqn.z += ms.x;
+    }
+    const float g0 = tm[idx];
+    gtp = make_float2(n.mE ? g0 : 0.0f, n.mS ? g0 : 0.0f);
+    if (X == 0) gtp.x += -g0;
+    if (X + 1 < W) gtp.x += -tm[idx + 1];
+    if (Y == 0) gtp.y += -g0;
+    if (Y + 1 < H) gtp.y += -tm[idx + W];
 }
 
 // Reverse of updateDualVariables (Python/vfs.cu:326-409) at a masked-in pixel.

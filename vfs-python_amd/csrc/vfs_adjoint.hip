@@ -82,9 +82,10 @@ __global__ void __launch_bounds__(256) k_adj_primal(float* __restrict__ gubB, fl
     float ga[3] = {gabc[3 * idx], gabc[3 * idx + 1], gabc[3 * idx + 2]};
     float e1 = geta[3 * idx + 1], e2 = geta[3 * idx + 2];
     float pa0 = gpp[GP_ALPHA0], pa1 = gpp[GP_ALPHA1];
+    float2 unused;
     primal_adj_pixel(n, q[idx], n.mW ? q[n.iW] : zero4, n.mN ? q[n.iN] : zero4, abc[3 * idx], abc[3 * idx + 1],
                      abc[3 * idx + 2], p[idx], eta[3 * idx + 1], eta[3 * idx + 2], tau, mu, alpha0, alpha1, g_ubo,
-                     g_vbo, ubB, vbB, uu, pp, qq, qm, ga, e1, e2, pa0, pa1);
+                     g_vbo, ubB, vbB, uu, pp, qq, qm, ga, e1, e2, pa0, pa1, unused);
     gubB[idx] = ubB;
     gvbB[idx] = vbB;
     gu[idx] = uu;
@@ -122,9 +123,10 @@ __global__ void __launch_bounds__(256) k_adj_l1(float* __restrict__ gu, float4* 
     float4 tm;
     float ubB = gubB[idx], e0 = geta[3 * idx], lam = gpar[GP_N * idx + GP_LAMBDA];
     float2 gd = make_float2(gderiv[2 * idx], gderiv[2 * idx + 1]);
+    float unused;
     const float g = l1_adj_pixel(n, gu[idx], tp_at(p, abc, mask, idx), tp_at(p, abc, mask, n.iW).x,
                                  tp_at(p, abc, mask, n.iN).y, deriv[idx], eta[3 * idx], u_prev[idx], u_bar_prev[idx],
-                                 tau, lambda, tm, ubB, e0, gd, lam);
+                                 tau, lambda, tm, ubB, e0, gd, lam, unused);
     gu[idx] = g;
     gubB[idx] = ubB;
     geta[3 * idx] = e0;

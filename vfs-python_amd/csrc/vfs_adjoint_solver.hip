@@ -13,6 +13,7 @@
 // without memsets (clearing is done by a fill kernel), so both functions can
 // be recorded into a HIP graph by stream capture.
 #include "vfs_adj_pixel.h"
+#include "vfs_internal.h"
 
 using namespace vfs;
 
@@ -77,7 +78,8 @@ struct AdjWs {
     float* tape;
 };
 
-// fused sweep tile (one 256-thread block) and its per-block parameter slots
+// fused sweep tile width and the most per-block parameter slots it can need
+// (tiles of 32 x 8, one row per thread)
 constexpr int FT_X = 32, FT_Y = 8;
 inline int64_t fused_blocks(int H, int W)
 {
@@ -186,20 +188,54 @@ __global__ void __launch_bounds__(256) k_warp_reset_save(float2* __restrict__ p,
     sflow[i] = flow[i];
 }
 
-// Checkpoint of the primal-dual state.
+// Checkpoint of the primal-dual state; with zero_pqv (the warp start of the blocked
+// path, whose kernel reads no p, q, v_bar there) p, q, v_bar are stored as zeros.
 __global__ void __launch_bounds__(256) k_save_state(float2* __restrict__ sp, float4* __restrict__ sq,
                                                     float* __restrict__ su, float* __restrict__ sub,
                                                     float2* __restrict__ svb, const float2* __restrict__ p,
                                                     const float4* __restrict__ q, const float* __restrict__ u,
-                                                    const float* __restrict__ ub, const float2* __restrict__ vb, int n)
+                                                    const float* __restrict__ ub, const float2* __restrict__ vb, int n,
+                                                    int zero_pqv)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    sp[i] = p[i];
-    sq[i] = q[i];
+    sp[i] = zero_pqv ? make_float2(0.0f, 0.0f) : p[i];
+    sq[i] = zero_pqv ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : q[i];
     su[i] = u[i];
     sub[i] = ub[i];
-    svb[i] = vb[i];
+    svb[i] = zero_pqv ? make_float2(0.0f, 0.0f) : vb[i];
+}
+
+// Warp-start record of the blocked path: u, u_bar and flow.
+__global__ void __launch_bounds__(256) k_warp_save(float* __restrict__ su, float* __restrict__ sub,
+                                                   float2* __restrict__ sflow, const float* __restrict__ u,
+                                                   const float* __restrict__ u_bar, const float2* __restrict__ flow,
+                                                   int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    su[i] = u[i];
+    sub[i] = u_bar[i];
+    sflow[i] = flow[i];
+}
+
+// computeOpticalFlowWithClamp with u0 = u in place (the blocked path keeps the
+// warp-start u in u; the thresholded iterate lives elsewhere).
+__global__ void __launch_bounds__(256) k_flow_clamp_inplace(float2* __restrict__ flow, float* __restrict__ u,
+                                                            const float* __restrict__ uf,
+                                                            const float2* __restrict__ tv,
+                                                            const uint8_t* __restrict__ mask, int n, float limit)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || mask[i] == 0) return;
+    const float base = u[i];
+    const float delta = fminf(uf[i] - base, limit);
+    u[i] = base + delta;
+    const float2 t = tv[i];
+    float2 f = flow[i];
+    f.x = fmaf(delta, t.x, f.x);
+    f.y = fmaf(delta, t.y, f.y);
+    flow[i] = f;
 }
 
 // Same as the solver's clamp: flow += delta tv, u = u0 + delta (Python/vfs.cu:641-676,
@@ -231,8 +267,9 @@ __global__ void __launch_bounds__(256) k_add(float* __restrict__ out, const floa
     if (i < n) out[i] += in[i];
 }
 
-// One step of the fused reverse sweep over a 32 x 8 tile.  With S_i the state before
-// iteration i (S_0 the warp start), launch F(i) runs
+// One step of the fused reverse sweep over a tile of 32 x 8R pixels (256 threads,
+// R rows each).  With S_i the state before iteration i (S_0 the warp start),
+// launch F(i) runs
 //   DO_D:   dual^T of iteration i (reads S_i: p, q, u_bar, v_bar) for the tile and a
 //           one-pixel ring, its u_bar / v_bar messages kept in LDS;
 //   DO_PL:  primal^T and threshold^T of iteration i - 1 (reads S_i: p, q and
@@ -240,19 +277,21 @@ __global__ void __launch_bounds__(256) k_add(float* __restrict__ out, const floa
 //   FINISH: instead of DO_PL, the adjoint of u_bar at the loop start (gub_out).
 // The head of a sweep is DO_PL alone (iteration n - 1, no messages yet), the tail
 // DO_D + FINISH (iteration 0).  Per-pixel arithmetic is that of the one-kernel-
-// per-step path; lambda / alpha shares are summed per block into double slots.
+// per-step path; the primal^T / threshold^T messages travel in compact form
+// (qmsg = (-g_S1, -g_S2), tmsg = g_g); lambda / alpha shares are summed per
+// block into double slots.
 struct FusedArgs {
     float* gu;
     float* gubB;
     float2* gvbB;
     const float2* gp;  // read (own pixel and, for the ring, the neighbouring tile's)
     const float4* gq;
-    const float4* qmsg;
-    const float4* tmsg;
+    const float2* qmsg;
+    const float* tmsg;
     float2* gp_out;  // written (own pixel)
     float4* gq_out;
-    float4* qmsg_out;
-    float4* tmsg_out;
+    float2* qmsg_out;
+    float* tmsg_out;
     float* gabc;
     float* geta;
     float2* gderiv;
@@ -269,12 +308,11 @@ struct FusedArgs {
     const float2* deriv;
     const uint8_t* mask;
     int H, W;
+    int nbx, nby;         // tiles
     float sigma;          // iteration i
     float tau_pl, mu_pl;  // iteration i - 1 (the head: n - 1)
     float alpha0, alpha1, eta_p, eta_q, lambda;
 };
-
-constexpr int FL_X = FT_X + 2, FL_Y = FT_Y + 2, F_RING = 2 * FL_X + 2 * FT_Y;
 
 // dual^T at pixel (X, Y) (in the image).  gab: adjoint of abc on entry (the owner's
 // accumulator, or zeros for a ring pixel), updated.
@@ -283,9 +321,6 @@ __device__ __forceinline__ void fused_dual(const FusedArgs& a, int X, int Y, flo
 {
     const int H = a.H, W = a.W, idx = Y * W + X;
     const Nbr n = nbr(a.mask, X, Y, H, W);
-    float4 gqn = a.gq[idx];
-    float2 gtp;
-    collect_qt(a.qmsg, a.tmsg, n, X, Y, H, W, idx, gqn, gtp);
     if (!n.m) {
         gpo = make_float2(0.0f, 0.0f);
         gqo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -293,6 +328,9 @@ __device__ __forceinline__ void fused_dual(const FusedArgs& a, int X, int Y, flo
         for (int k = 0; k < 6; ++k) msg.vm[k] = 0.0f;
         return;
     }
+    float4 gqn = a.gq[idx];
+    float2 gtp;
+    collect_qt_compact(a.qmsg, a.tmsg, n, X, Y, H, W, idx, gqn, gtp);
     const bool fx = n.mE, fy = n.mS;
     const int ix = fx ? n.iE : n.iW, iy = fy ? n.iS : n.iN;
     dual_adj_pixel(fx, fy, a.ub1[idx], a.vb1[idx], a.ub1[ix], a.vb1[ix], a.ub1[iy], a.vb1[iy], a.abc[3 * idx],
@@ -300,12 +338,13 @@ __device__ __forceinline__ void fused_dual(const FusedArgs& a, int X, int Y, flo
                    a.eta_q, a.gp[idx], gqn, gtp, gpo, gqo, msg, gab, gpa0, gpa1);
 }
 
+template <int LX>
 struct LdsMsgs {
-    const float* m;  // FL_Y x FL_X x 9
+    const float* m;  // rows x LX x 9
     int X0, Y0;
     __device__ __forceinline__ DualMsg operator()(int x, int y) const
     {
-        const float* r = m + 9 * ((y - Y0 + 1) * FL_X + (x - X0 + 1));
+        const float* r = m + 9 * ((y - Y0 + 1) * LX + (x - X0 + 1));
         DualMsg d;
         for (int k = 0; k < 3; ++k) d.um[k] = r[k];
         for (int k = 0; k < 6; ++k) d.vm[k] = r[3 + k];
@@ -320,46 +359,72 @@ __device__ __forceinline__ void put_msg(float* m, int l, const DualMsg& d)
     for (int k = 0; k < 6; ++k) r[3 + k] = d.vm[k];
 }
 
-template <bool DO_D, bool DO_PL, bool FINISH>
+// Tile of a block.  Workgroups are dispatched to the eight XCDs in turn (block b
+// to XCD b mod 8); tiles are handed out so that each XCD works on one contiguous
+// run of tiles in row-major order, which keeps the halo rows a tile shares with
+// its neighbours in that XCD's L2.
+__device__ __forceinline__ void tile_of(int b, int nbx, int nby, int& bx, int& by)
+{
+    const int n = nbx * nby;
+    const int x = b & 7, k = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; ++y) start += (n - y + 7) >> 3;
+    const int t = start + k;
+    bx = t % nbx;
+    by = t / nbx;
+}
+
+template <int R, bool DO_D, bool DO_PL, bool FINISH>
 __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
 {
-    __shared__ float s_msg[FL_Y * FL_X * 9];
+    constexpr int TX = FT_X, TY = 8 * R, LX = TX + 2, LY = TY + 2, RING = 2 * LX + 2 * TY;
+    __shared__ float s_msg[DO_D ? LY * LX * 9 : 1];
     __shared__ float s_red[3 * 4];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FT_X + tx;
-    const int X0 = blockIdx.x * FT_X, Y0 = blockIdx.y * FT_Y;
-    const int X = X0 + tx, Y = Y0 + ty;
+    const int tid = threadIdx.x, tx = tid & (TX - 1), ty = tid / TX;
+    int bx, by;
+    tile_of(blockIdx.x, a.nbx, a.nby, bx, by);
+    const int X0 = bx * TX, Y0 = by * TY;
     const int H = a.H, W = a.W;
-    const bool inside = X < W && Y < H;
-    const int idx = inside ? Y * W + X : 0;
+    const int X = X0 + tx;
     float gpl = 0.0f, gpa0 = 0.0f, gpa1 = 0.0f;
-    float2 gpo = make_float2(0.0f, 0.0f);
-    float4 gqo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float gab[3] = {0.0f, 0.0f, 0.0f};
-    if (inside) {
-        gab[0] = a.gabc[3 * idx];
-        gab[1] = a.gabc[3 * idx + 1];
-        gab[2] = a.gabc[3 * idx + 2];
+    float2 gpo[R];
+    float4 gqo[R];
+    float gab[R][3];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int Y = Y0 + ty + 8 * r;
+        gpo[r] = make_float2(0.0f, 0.0f);
+        gqo[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 3; ++k) gab[r][k] = 0.0f;
+        if (X < W && Y < H) {
+            const int idx = Y * W + X;
+            for (int k = 0; k < 3; ++k) gab[r][k] = a.gabc[3 * idx + k];
+        }
     }
     if constexpr (DO_D) {
-        if (inside) {
-            DualMsg m;
-            fused_dual(a, X, Y, gpo, gqo, m, gab, gpa0, gpa1);
-            put_msg(s_msg, (ty + 1) * FL_X + tx + 1, m);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int Y = Y0 + ty + 8 * r;
+            if (X < W && Y < H) {
+                DualMsg m;
+                fused_dual(a, X, Y, gpo[r], gqo[r], m, gab[r], gpa0, gpa1);
+                put_msg(s_msg, (ty + 8 * r + 1) * LX + tx + 1, m);
+            }
         }
-        if (tid < F_RING) {
+        for (int t = tid; t < RING; t += 256) {
             int lx, ly;
-            if (tid < FL_X) {
-                lx = tid;
+            if (t < LX) {
+                lx = t;
                 ly = 0;
-            } else if (tid < 2 * FL_X) {
-                lx = tid - FL_X;
-                ly = FL_Y - 1;
-            } else if (tid < 2 * FL_X + FT_Y) {
+            } else if (t < 2 * LX) {
+                lx = t - LX;
+                ly = LY - 1;
+            } else if (t < 2 * LX + TY) {
                 lx = 0;
-                ly = 1 + tid - 2 * FL_X;
+                ly = 1 + t - 2 * LX;
             } else {
-                lx = FL_X - 1;
-                ly = 1 + tid - 2 * FL_X - FT_Y;
+                lx = LX - 1;
+                ly = 1 + t - 2 * LX - TY;
             }
             const int gx = X0 - 1 + lx, gy = Y0 - 1 + ly;
             if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
@@ -369,64 +434,68 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
                 float d0 = 0.0f, d1 = 0.0f;
                 DualMsg m;
                 fused_dual(a, gx, gy, p_, q_, m, g_, d0, d1);
-                put_msg(s_msg, ly * FL_X + lx, m);
+                put_msg(s_msg, ly * LX + lx, m);
             }
         }
         __syncthreads();
     }
-    if (inside) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int Y = Y0 + ty + 8 * r;
+        if (X >= W || Y >= H) continue;
+        const int idx = Y * W + X;
         if constexpr (DO_PL) {
             float g_ubo = a.gubB[idx];
             float2 g_vbo = a.gvbB[idx];
-            if constexpr (DO_D) gather_msgs(a.mask, LdsMsgs{s_msg, X0, Y0}, X, Y, H, W, g_ubo, g_vbo);
+            if constexpr (DO_D) gather_msgs(a.mask, LdsMsgs<LX>{s_msg, X0, Y0}, X, Y, H, W, g_ubo, g_vbo);
             const Nbr n = nbr(a.mask, X, Y, H, W);
             if (!n.m) {
                 a.gubB[idx] = 0.0f;
                 a.gvbB[idx] = make_float2(0.0f, 0.0f);
-                a.qmsg_out[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                a.qmsg_out[idx] = make_float2(0.0f, 0.0f);
                 a.gu[idx] = 0.0f;
-                a.tmsg_out[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                a.gp_out[idx] = DO_D ? gpo : a.gp[idx];
-                a.gq_out[idx] = DO_D ? gqo : a.gq[idx];
+                a.tmsg_out[idx] = 0.0f;
+                a.gp_out[idx] = DO_D ? gpo[r] : a.gp[idx];
+                a.gq_out[idx] = DO_D ? gqo[r] : a.gq[idx];
             } else {
                 const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                float2 gp_ = DO_D ? gpo : a.gp[idx];
-                float4 gq_ = DO_D ? gqo : a.gq[idx];
+                float2 gp_ = DO_D ? gpo[r] : a.gp[idx];
+                float4 gq_ = DO_D ? gqo[r] : a.gq[idx];
                 float gu_ = a.gu[idx];
                 float e0 = a.geta[3 * idx], e1 = a.geta[3 * idx + 1], e2 = a.geta[3 * idx + 2];
-                float ubB;
-                float2 vbB;
+                float ubB, g_g;
+                float2 vbB, qmc;
                 float4 qm, tm;
                 const float ea = a.abc[3 * idx], eb = a.abc[3 * idx + 1], ec = a.abc[3 * idx + 2];
                 primal_adj_pixel(n, a.q1[idx], n.mW ? a.q1[n.iW] : zero4, n.mN ? a.q1[n.iN] : zero4, ea, eb, ec,
                                  a.p1[idx], a.eta[3 * idx + 1], a.eta[3 * idx + 2], a.tau_pl, a.mu_pl, a.alpha0,
-                                 a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_, qm, gab, e1, e2, gpa0, gpa1);
+                                 a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_, qm, gab[r], e1, e2, gpa0, gpa1, qmc);
                 float2 gd = a.gderiv[idx];
                 gu_ = l1_adj_pixel(n, gu_, tp_at(a.p1, a.abc, a.mask, idx), tp_at(a.p1, a.abc, a.mask, n.iW).x,
                                    tp_at(a.p1, a.abc, a.mask, n.iN).y, a.deriv[idx], a.eta[3 * idx], a.u2[idx],
-                                   a.ub2[idx], a.tau_pl, a.lambda, tm, ubB, e0, gd, gpl);
+                                   a.ub2[idx], a.tau_pl, a.lambda, tm, ubB, e0, gd, gpl, g_g);
                 a.gu[idx] = gu_;
                 a.gubB[idx] = ubB;
                 a.gvbB[idx] = vbB;
                 a.gp_out[idx] = gp_;
                 a.gq_out[idx] = gq_;
-                a.qmsg_out[idx] = qm;
-                a.tmsg_out[idx] = tm;
+                a.qmsg_out[idx] = qmc;
+                a.tmsg_out[idx] = g_g;
                 a.geta[3 * idx] = e0;
                 a.geta[3 * idx + 1] = e1;
                 a.geta[3 * idx + 2] = e2;
                 a.gderiv[idx] = gd;
-                for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[k];
+                for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[r][k];
             }
         } else {
             // DO_D alone: the dual^T outputs of the owned pixel
-            a.gp_out[idx] = gpo;
-            a.gq_out[idx] = gqo;
-            for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[k];
+            a.gp_out[idx] = gpo[r];
+            a.gq_out[idx] = gqo[r];
+            for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[r][k];
             if constexpr (FINISH) {
                 float g = a.gubB[idx];
                 float2 gv = a.gvbB[idx];
-                gather_msgs(a.mask, LdsMsgs{s_msg, X0, Y0}, X, Y, H, W, g, gv);
+                gather_msgs(a.mask, LdsMsgs<LX>{s_msg, X0, Y0}, X, Y, H, W, g, gv);
                 a.gub_out[idx] = g;
             }
         }
@@ -440,7 +509,7 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
         for (int k = 0; k < 3; ++k) s_red[3 * (tid >> 6) + k] = v[k];
     __syncthreads();
     if (tid == 0) {
-        double* slot = a.gpart + 3 * (static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x);
+        double* slot = a.gpart + 3 * (static_cast<int64_t>(by) * a.nbx + bx);
         for (int k = 0; k < 3; ++k)
             slot[k] += static_cast<double>(s_red[k]) + static_cast<double>(s_red[3 + k]) +
                        static_cast<double>(s_red[6 + k]) + static_cast<double>(s_red[9 + k]);
@@ -450,18 +519,44 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
 // Adds each fused tile's (lambda, alpha0, alpha1) sums to the per-pixel shares of
 // the tile's first pixel.
 __global__ void __launch_bounds__(256) k_fold_gpart(float* __restrict__ gpar, const double* __restrict__ gpart,
-                                                    int H, int W, int nbx, int nblocks)
+                                                    int W, int nbx, int tile_y, int nblocks)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblocks) return;
-    const int X = (b % nbx) * FT_X, Y = (b / nbx) * FT_Y;
+    const int X = (b % nbx) * FT_X, Y = (b / nbx) * tile_y;
     float* g = gpar + static_cast<int64_t>(GP_N) * (static_cast<int64_t>(Y) * W + X);
     g[GP_LAMBDA] += static_cast<float>(gpart[3 * b]);
     g[GP_ALPHA0] += static_cast<float>(gpart[3 * b + 1]);
     g[GP_ALPHA1] += static_cast<float>(gpart[3 * b + 2]);
 }
 
+// Rows per thread of the fused sweep at a level of n pixels.  Measured on MI355X at
+// 800 x 800 / 400 x 400 / 200 x 200: one row (the most resident waves) is the
+// fastest at every level; the longer tiles trade occupancy for less ring work
+// and lose.
+inline int fused_rows(int64_t) { return 1; }
+
 inline unsigned nb1(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+// kind 0: head (PL), 1: D + PL, 2: tail (D + finish)
+template <int R>
+int launch_fused(int kind, int nblocks, hipStream_t s, const FusedArgs& fa)
+{
+    if (kind == 0)
+        hipLaunchKernelGGL((k_adj_fused<R, false, true, false>), dim3(nblocks), dim3(256), 0, s, fa);
+    else if (kind == 1)
+        hipLaunchKernelGGL((k_adj_fused<R, true, true, false>), dim3(nblocks), dim3(256), 0, s, fa);
+    else
+        hipLaunchKernelGGL((k_adj_fused<R, true, false, true>), dim3(nblocks), dim3(256), 0, s, fa);
+    return hipGetLastError() == hipSuccess ? 0 : VFS_EINVAL;
+}
+
+int launch_fused_rows(int rows, int kind, int nblocks, hipStream_t s, const FusedArgs& fa)
+{
+    if (rows == 4) return launch_fused<4>(kind, nblocks, s, fa);
+    if (rows == 2) return launch_fused<2>(kind, nblocks, s, fa);
+    return launch_fused<1>(kind, nblocks, s, fa);
+}
 
 #define VFS_TRY(expr)             \
     do {                          \
@@ -484,7 +579,8 @@ int check_args(const vfs_level* lv, const vfs_params* prm, const float* taus, co
     return 0;
 }
 
-// State S_{i+1} from S_i (iteration i of the schedule), written out of place.
+// State S_{i+1} from S_i (iteration i of the schedule), written out of place by the
+// per-kernel steps.
 int pd_step(void* st, const vfs_level* lv, const vfs_params* prm, const State& a, const State& b, float* tp, float tau,
             float sigma, float mu)
 {
@@ -521,15 +617,72 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt)
 }
 
 int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
-                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save)
+                          const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save,
+                          float* workspace, const vfs_adj_tiles* tiles)
 {
     VFS_TRY(check_args(lv, prm, taus, sigmas, mus, n_iter, n_warp, ckpt));
     if (save == nullptr && n_warp > 0) return VFS_EINVAL;
+    const int fk = tiles ? tiles->fwd_k : 0;
+    if (fk > 0 && workspace == nullptr) return VFS_EINVAL;
     const int H = lv->H, W = lv->W;
     const int64_t n = static_cast<int64_t>(H) * W;
     hipStream_t s = S(stream);
     VFS_TRY(vfs_gaussian_masked(stream, lv->ref_smoothed, lv->ref, lv->mask, H, W));
     VFS_TRY(vfs_compute_tensor_eta(stream, lv->abc, lv->eta, lv->ref_smoothed, lv->mask, H, W, prm));
+    if (fk > 0) {
+        // blocked path: k-iteration launches between two state sets, with a launch
+        // boundary at every checkpoint; set 1 is the first tape slot of the workspace
+        const AdjWs A = adj_ws_at(workspace, n, fused_blocks(H, W));
+        const State t1 = state_at(A.tape, n);
+        float* su[2] = {lv->u0, t1.u};
+        float* sub[2] = {lv->u_bar, t1.ub};
+        float* svb[2] = {lv->v_bar, t1.vb};
+        float* sp[2] = {lv->p, t1.p};
+        float* sq[2] = {lv->q, t1.q};
+        int cur = 0;  // set holding u_bar
+        for (int w = 0; w < n_warp; ++w) {
+            const WarpSave ws = warp_save_at(save, n, n_iter, ckpt, w);
+            VFS_TRY(vfs_find_warping_vector(stream, lv->tv, lv->flow, H, W, lv->tvf, H, W));
+            VFS_TRY(vfs_warp_image(stream, lv->warped, lv->flow, H, W, lv->other, H, W));
+            VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+            hipLaunchKernelGGL(k_warp_save, dim3(nb1(n)), dim3(256), 0, s, ws.u, ws.ub,
+                               reinterpret_cast<float2*>(ws.flow), lv->u, sub[cur],
+                               reinterpret_cast<const float2*>(lv->flow), static_cast<int>(n));
+            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            int done = 0;
+            bool first = true;
+            while (done < n_iter) {
+                if (done % ckpt == 0) {
+                    const State c = state_at(ws.ckpt0 + static_cast<int64_t>(done / ckpt) * state_floats(n), n);
+                    hipLaunchKernelGGL(k_save_state, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(c.p),
+                                       reinterpret_cast<float4*>(c.q), c.u, c.ub, reinterpret_cast<float2*>(c.vb),
+                                       reinterpret_cast<const float2*>(sp[cur]),
+                                       reinterpret_cast<const float4*>(sq[cur]), first ? lv->u : su[cur], sub[cur],
+                                       reinterpret_cast<const float2*>(svb[cur]), static_cast<int>(n), first ? 1 : 0);
+                    if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+                }
+                int kc = n_iter - done < fk ? n_iter - done : fk;
+                const int to_ck = ckpt - done % ckpt;
+                if (kc > to_ck) kc = to_ck;
+                VFS_TRY(vfs_pd_blocks(stream, H, W, lv->mask, lv->abc, lv->eta, lv->deriv, first ? lv->u : su[cur],
+                                      sub[cur], svb[cur], sp[cur], sq[cur], su[1 - cur], sub[1 - cur], svb[1 - cur],
+                                      sp[1 - cur], sq[1 - cur], first ? 1 : 0, taus + done, sigmas + done, mus + done,
+                                      kc, tiles->fwd_tile_x, tiles->fwd_tile_y, fk, prm));
+                cur = 1 - cur;
+                done += kc;
+                first = false;
+            }
+            const float* uth = n_iter > 0 ? su[cur] : lv->u;
+            VFS_TRY(vfs_median_filter(stream, lv->uf, uth, lv->mask, H, W, prm->median_k));
+            hipLaunchKernelGGL(k_flow_clamp_inplace, dim3(nb1(n)), dim3(256), 0, s,
+                               reinterpret_cast<float2*>(lv->flow), lv->u, lv->uf,
+                               reinterpret_cast<const float2*>(lv->tv), lv->mask, static_cast<int>(n),
+                               prm->limit_range);
+            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        }
+        if (cur != 0) VFS_TRY(vfs_copy(stream, lv->u_bar, sub[cur], n));
+        return 0;
+    }
     for (int w = 0; w < n_warp; ++w) {
         const WarpSave ws = warp_save_at(save, n, n_iter, ckpt, w);
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, lv->flow, H, W, lv->tvf, H, W));
@@ -546,7 +699,8 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
                 hipLaunchKernelGGL(k_save_state, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(c.p),
                                    reinterpret_cast<float4*>(c.q), c.u, c.ub, reinterpret_cast<float2*>(c.vb),
                                    reinterpret_cast<const float2*>(lv->p), reinterpret_cast<const float4*>(lv->q),
-                                   lv->u, lv->u_bar, reinterpret_cast<const float2*>(lv->v_bar), static_cast<int>(n));
+                                   lv->u, lv->u_bar, reinterpret_cast<const float2*>(lv->v_bar), static_cast<int>(n),
+                                   0);
                 if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
             }
             VFS_TRY(vfs_update_dual(stream, lv->p, lv->tp, lv->q, sigmas[it], lv->u_bar, lv->v_bar, lv->abc, lv->mask,
@@ -567,14 +721,19 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
 
 int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
                           const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, const float* save,
-                          float* workspace, float* gu, float* gub, float* gf, float* gref, float* gpar, int mode)
+                          float* workspace, float* gu, float* gub, float* gf, float* gref, float* gpar,
+                          const vfs_adj_tiles* tiles)
 {
+    const int mode = tiles ? tiles->sweep : 0;
+    const int rtx = tiles ? tiles->rec_tile_x : 0, rty = tiles ? tiles->rec_tile_y : 0;
+    const int rk = tiles ? tiles->rec_k : 0;
+    if (rtx > 0 && rk < 1) return VFS_EINVAL;
     VFS_TRY(check_args(lv, prm, taus, sigmas, mus, n_iter, n_warp, ckpt));
     const void* ptrs[] = {workspace, gu, gub, gf, gref, gpar};
     for (const void* p : ptrs)
         if (p == nullptr) return VFS_EINVAL;
     if (save == nullptr && n_warp > 0) return VFS_EINVAL;
-    if (mode != 0 && mode != 1) return VFS_EINVAL;
+    if (mode != 0 && mode != 1 && mode != 11 && mode != 12 && mode != 14) return VFS_EINVAL;
     const int H = lv->H, W = lv->W;
     const int64_t n = static_cast<int64_t>(H) * W;
     hipStream_t s = S(stream);
@@ -606,7 +765,11 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     fa.eta_p = prm->eta_p;
     fa.eta_q = prm->eta_q;
     fa.lambda = prm->lambda;
-    const dim3 fgrid(nbx, nby), fblock(FT_X, FT_Y);
+    const int rows = mode == 0 ? fused_rows(n) : mode - 10;
+    const int tile_y = FT_Y * rows, fnby = (H + tile_y - 1) / tile_y, fblocks = nbx * fnby;
+    fa.nbx = nbx;
+    fa.nby = fnby;
+    const bool fused = mode != 1;
     for (int w = n_warp - 1; w >= 0; --w) {
         const WarpSave ws = warp_save_at(sv, n, n_iter, ckpt, w);
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, ws.flow, H, W, lv->tvf, H, W));
@@ -618,11 +781,49 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
             if (i == a) return state_at(ws.ckpt0 + static_cast<int64_t>(seg) * state_floats(n), n);
             return state_at(A.tape + static_cast<int64_t>(i - a - 1) * state_floats(n), n);
         };
+        // states a + 1 .. b of segment seg into the tape: blocked launches of up to rk
+        // iterations that also write their intermediate states, or per-kernel steps
         auto recompute = [&](int seg) {
             const int a = seg * ckpt, b = (seg + 1) * ckpt < n_iter ? (seg + 1) * ckpt : n_iter;
-            for (int i = a; i < b; ++i)
-                VFS_TRY(pd_step(stream, lv, prm, state_of(seg, i), state_of(seg, i + 1), A.tp, taus[i], sigmas[i],
-                                mus[i]));
+            if (rtx <= 0) {
+                for (int i = a; i < b; ++i)
+                    VFS_TRY(pd_step(stream, lv, prm, state_of(seg, i), state_of(seg, i + 1), A.tp, taus[i],
+                                    sigmas[i], mus[i]));
+                return 0;
+            }
+            for (int i = a; i < b;) {
+                const int kc = b - i < rk ? b - i : rk;
+                const State in = state_of(seg, i), out = state_of(seg, i + kc);
+                PdTapeLaunch L;
+                L.H = H;
+                L.W = W;
+                L.mask = lv->mask;
+                L.abc = lv->abc;
+                L.eta = lv->eta;
+                L.deriv = lv->deriv;
+                const float* ins[5] = {in.p, in.q, in.u, in.ub, in.vb};
+                float* outs[5] = {out.p, out.q, out.u, out.ub, out.vb};
+                for (int c = 0; c < 5; ++c) {
+                    L.in[c] = ins[c];
+                    L.out[c] = outs[c];
+                }
+                L.taus = taus + i;
+                L.sigmas = sigmas + i;
+                L.mus = mus + i;
+                L.n_iter = kc;
+                L.tile_x = rtx;
+                L.tile_y = rty;
+                L.k = rk;
+                L.tape = kc > 1 ? state_of(seg, i + 1).p : nullptr;
+                L.tape_stride = state_floats(n);
+                L.tape_off[0] = al(2 * n);
+                L.tape_off[1] = al(2 * n) + al(4 * n);
+                L.tape_off[2] = L.tape_off[1] + al(n);
+                L.tape_off[3] = L.tape_off[2] + al(n);
+                L.tape_count = kc - 1;
+                VFS_TRY(pd_blocks_tape(s, L, prm));
+                i += kc;
+            }
             return 0;
         };
         const float* u_th = ws.u;
@@ -638,7 +839,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         VFS_TRY(vfs_fill(stream, A.gp, 0.0f, 2 * n));
         VFS_TRY(vfs_fill(stream, A.gq, 0.0f, 4 * n));
         VFS_TRY(vfs_fill(stream, A.gderiv, 0.0f, 2 * n));
-        if (nck > 0 && mode == 0) {
+        if (nck > 0 && fused) {
             // fused sweep: head PL(n-1), F(i) for i = n-1 .. 1, tail D(0) + finish
             int cs = nck - 1;
             // ping-pong of (gp, gq, qmsg, tmsg): set 0 is A.gp ..., set 1 the second copies;
@@ -651,12 +852,12 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                 float* tmi[2] = {A.tmsg, A.tmsg2};
                 fa.gp = reinterpret_cast<const float2*>(gpi[cur]);
                 fa.gq = reinterpret_cast<const float4*>(gqi[cur]);
-                fa.qmsg = reinterpret_cast<const float4*>(qmi[cur]);
-                fa.tmsg = reinterpret_cast<const float4*>(tmi[cur]);
+                fa.qmsg = reinterpret_cast<const float2*>(qmi[cur]);
+                fa.tmsg = tmi[cur];
                 fa.gp_out = reinterpret_cast<float2*>(gpi[1 - cur]);
                 fa.gq_out = reinterpret_cast<float4*>(gqi[1 - cur]);
-                fa.qmsg_out = reinterpret_cast<float4*>(qmi[1 - cur]);
-                fa.tmsg_out = reinterpret_cast<float4*>(tmi[1 - cur]);
+                fa.qmsg_out = reinterpret_cast<float2*>(qmi[1 - cur]);
+                fa.tmsg_out = tmi[1 - cur];
                 cur = 1 - cur;
             };
             auto set_states = [&](const State& si, const State& sim1) {
@@ -672,8 +873,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
             fa.tau_pl = taus[n_iter - 1];
             fa.mu_pl = mus[n_iter - 1];
             set_bufs();
-            hipLaunchKernelGGL((k_adj_fused<false, true, false>), fgrid, fblock, 0, s, fa);
-            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            VFS_TRY(launch_fused_rows(rows, 0, fblocks, s, fa));
             for (int i = n_iter - 1; i >= 1; --i) {
                 if (i - 1 < cs * ckpt) {
                     --cs;
@@ -684,15 +884,13 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                 fa.tau_pl = taus[i - 1];
                 fa.mu_pl = mus[i - 1];
                 set_bufs();
-                hipLaunchKernelGGL((k_adj_fused<true, true, false>), fgrid, fblock, 0, s, fa);
-                if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+                VFS_TRY(launch_fused_rows(rows, 1, fblocks, s, fa));
             }
             const State s0 = state_at(ws.ckpt0, n);
             set_states(s0, s0);
             fa.sigma = sigmas[0];
             set_bufs();
-            hipLaunchKernelGGL((k_adj_fused<true, false, true>), fgrid, fblock, 0, s, fa);
-            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            VFS_TRY(launch_fused_rows(rows, 2, fblocks, s, fa));
         } else if (nck > 0) {
             // one kernel per step (primal^T, threshold^T, dual^T), messages through memory
             int has_msgs = 0;
@@ -742,9 +940,9 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         VFS_TRY(vfs_adj_derivatives(stream, gref, A.gow, A.gderiv, lv->tvf, H, W));
         VFS_TRY(vfs_adj_warp(stream, gf, A.gow, A.gtv, ws.flow, lv->other, lv->tvf, H, W));
     }
-    if (n_warp > 0 && nck > 0 && mode == 0) {
-        hipLaunchKernelGGL(k_fold_gpart, dim3(nb1(nblocks)), dim3(256), 0, s, gpar, A.gpart, H, W, nbx,
-                           static_cast<int>(nblocks));
+    if (n_warp > 0 && nck > 0 && fused) {
+        hipLaunchKernelGGL(k_fold_gpart, dim3(nb1(fblocks)), dim3(256), 0, s, gpar, A.gpart, W, nbx, tile_y,
+                           fblocks);
         if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
     }
     VFS_TRY(vfs_fill(stream, A.gIs, 0.0f, n));

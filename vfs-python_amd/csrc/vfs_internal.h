@@ -1,0 +1,33 @@
+// Entry points shared between the library's translation units (not part of the
+// C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vfs_hip.h"
+
+namespace vfs {
+
+// One launch of the blocked primal-dual kernel (tile_x x tile_y, k iterations per
+// launch at most) that runs n_iter iterations from the state `in` to the state
+// `out` and also writes the state after each of the first tape_count iterations
+// into tape + i * tape_stride.  State arrays: p (H,W,2), q (H,W,4), u, u_bar (H,W),
+// v_bar (H,W,2); tape_off = offsets (floats) of q, u, u_bar, v_bar in a tape slot
+// (p at 0).  taus / sigmas / mus: host arrays of n_iter values.
+struct PdTapeLaunch {
+    int H, W;
+    const uint8_t* mask;
+    const float *abc, *eta, *deriv;
+    const float* in[5];  // p, q, u, u_bar, v_bar
+    float* out[5];
+    const float *taus, *sigmas, *mus;
+    int n_iter, tile_x, tile_y, k;
+    float* tape;
+    int64_t tape_stride, tape_off[4];
+    int tape_count;
+};
+
+int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm);
+
+}  // namespace vfs

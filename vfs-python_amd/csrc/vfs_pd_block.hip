@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "vfs_device.h"
+#include "vfs_internal.h"
 #include "vfs_pd.h"
 
 using namespace vfs;
@@ -58,6 +59,12 @@ struct PDLevel {
     float* vb_out;
     float* p_out;
     float* q_out;
+    // Optional: the tile's state after each of the first tape_count iterations of
+    // the launch, into tape + k * tape_stride (p at offset 0, then q, u, u_bar, v_bar
+    // at the given offsets, in floats).  Used to recompute a tape for the adjoint.
+    float* tape = nullptr;
+    int64_t tape_stride = 0, tape_q = 0, tape_u = 0, tape_ub = 0, tape_vb = 0;
+    int tape_count = 0;
 };
 
 enum : unsigned {
@@ -72,6 +79,32 @@ enum : unsigned {
     F_HN = 256u,
     F_IMG = 512u,   // inside the image
 };
+
+// The tile pixels' state into tape slot k (see PDLevel::tape).
+template <int TX, int TY, int K, int NT, int S>
+__device__ __forceinline__ void pd_tape_write(const PDLevel& L, int k, int ox, int oy, const unsigned (&fl)[S],
+                                              const float (&u)[S], const float (&ub)[S], const float (&vbx)[S],
+                                              const float (&vby)[S], const float (&px)[S], const float (&py)[S],
+                                              const float (&qx)[S], const float (&qy)[S], const float (&qz)[S],
+                                              const float (&qw)[S])
+{
+    constexpr int EX = TX + 3 * K, EY = TY + 3 * K, E = EX * EY;
+    float* base = L.tape + static_cast<int64_t>(k) * L.tape_stride;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int e = threadIdx.x + NT * j;
+        if (e >= E) continue;
+        const int ex = e % EX, ey = e / EX;
+        if (ex < 2 * K || ex >= 2 * K + TX || ey < 2 * K || ey >= 2 * K + TY) continue;
+        if (!(fl[j] & F_IMG)) continue;
+        const int64_t idx = static_cast<int64_t>(oy + ey) * L.W + (ox + ex);
+        reinterpret_cast<float2*>(base)[idx] = make_float2(px[j], py[j]);
+        reinterpret_cast<float4*>(base + L.tape_q)[idx] = make_float4(qx[j], qy[j], qz[j], qw[j]);
+        base[L.tape_u + idx] = u[j];
+        base[L.tape_ub + idx] = ub[j];
+        reinterpret_cast<float2*>(base + L.tape_vb)[idx] = make_float2(vbx[j], vby[j]);
+    }
+}
 
 template <int TX, int TY, int K, int NT>
 __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, int k_count, const PDSched& sc,
@@ -227,6 +260,13 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
             sA[E + e] = vbx[j];
             sA[2 * E + e] = vby[j];
         }
+        if (L.tape != nullptr && k < L.tape_count)
+            pd_tape_write<TX, TY, K, NT>(L, k, ox, oy, fl, u, ub, vbx, vby, px, py, qx, qy, qz, qw);
+    }
+    if (skip && L.tape != nullptr) {
+        // a fully masked region: every iteration's state is zero
+        for (int k = 0; k < k_count && k < L.tape_count; ++k)
+            pd_tape_write<TX, TY, K, NT>(L, k, ox, oy, fl, u, ub, vbx, vby, px, py, qx, qy, qz, qw);
     }
 
     // write back the tile interior
@@ -566,6 +606,51 @@ int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* 
     const PDSched sc = make_sched(prm, taus, sigmas, mus, n_iter);
     return launch_pd_single(S(stream), v, L, n_iter, sc, make_consts(prm));
 }
+
+}  // extern "C"
+
+namespace vfs {
+
+int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm)
+{
+    VFS_CHECK_DIMS(a.H, a.W);
+    if (prm == nullptr || a.n_iter < 1 || a.n_iter > a.k || a.tape_count < 0 || a.tape_count >= a.n_iter)
+        return VFS_EINVAL;
+    if (a.tape_count > 0 && a.tape == nullptr) return VFS_EINVAL;
+    const Variant* v = find_variant(a.tile_x, a.tile_y, a.k);
+    if (v == nullptr) return VFS_EINVAL;
+    PDLevel L;
+    L.H = a.H;
+    L.W = a.W;
+    L.first = 0;
+    L.mask = a.mask;
+    L.abc = a.abc;
+    L.eta = a.eta;
+    L.deriv = a.deriv;
+    L.p_in = a.in[0];
+    L.q_in = a.in[1];
+    L.u_in = a.in[2];
+    L.ub_in = a.in[3];
+    L.vb_in = a.in[4];
+    L.p_out = a.out[0];
+    L.q_out = a.out[1];
+    L.u_out = a.out[2];
+    L.ub_out = a.out[3];
+    L.vb_out = a.out[4];
+    L.tape = a.tape;
+    L.tape_stride = a.tape_stride;
+    L.tape_q = a.tape_off[0];
+    L.tape_u = a.tape_off[1];
+    L.tape_ub = a.tape_off[2];
+    L.tape_vb = a.tape_off[3];
+    L.tape_count = a.tape_count;
+    const PDSched sc = make_sched(prm, a.taus, a.sigmas, a.mus, a.n_iter);
+    return launch_pd_single(s, v, L, a.n_iter, sc, make_consts(prm));
+}
+
+}  // namespace vfs
+
+extern "C" {
 
 int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* ws, const vfs_params* prm,
                             const float* taus, const float* sigmas, const float* mus, int n_iter, int n_warp,

@@ -46,11 +46,23 @@ def _e(shape, ch=None):
 
 
 class VFSDepthAdjoint:
-    def __init__(self, param, mask, tvf, cvf, cam, t, z_gt, checkpoint=25, fused=True):
+    def __init__(self, param, mask, tvf, cvf, cam, t, z_gt, checkpoint=25, fused=True, rows=None, blocked=True,
+                 rec_tile="auto"):
         import vfs as V
         self.param = copy.copy(param)
         self.checkpoint = int(checkpoint)
         self.fused = bool(fused)
+        if rows not in (None, 1, 2, 4):
+            raise ValueError("rows must be None, 1, 2 or 4")
+        # reverse sweep: 1 = one kernel per step; 0 = fused, rows per thread by level size;
+        # 10 + rows = fused with that many rows per thread
+        self._mode = 1 if not self.fused else (0 if rows is None else 10 + rows)
+        # forward: blocked primal-dual launches (variant per level from the autotune table)
+        # or the per-kernel steps; tape recompute: blocked launches that also write
+        # their intermediate states, with the forward's variant ("auto") or a given
+        # (tile_x, tile_y, k), or the per-kernel steps (None)
+        self.blocked = bool(blocked)
+        self.rec_tile = rec_tile if rec_tile in (None, "auto") else tuple(int(x) for x in rec_tile)
         if self.checkpoint < 1:
             raise ValueError("checkpoint must be >= 1")
         self.v = V.VFS(self.param)
@@ -80,6 +92,18 @@ class VFSDepthAdjoint:
         n_iter, n_warp = int(p.nSolverIters), int(p.nWarpIters)
         taus, sigmas, mus = (np.ascontiguousarray(a, dtype=np.float32) for a in H.pd_schedule(n_iter, p.timeStepLambda))
         return n_iter, n_warp, taus, sigmas, mus
+
+    def _tiles(self, k):
+        import autotune
+        t = H.VfsAdjTiles()
+        hh, ww = self.sizes[k]
+        best = autotune.tile_for(hh, ww, int(self.param.nSolverIters))
+        if self.blocked:
+            t.fwd_tile_x, t.fwd_tile_y, t.fwd_k = best
+        if self.rec_tile is not None:
+            t.rec_tile_x, t.rec_tile_y, t.rec_k = best if self.rec_tile == "auto" else self.rec_tile
+        t.sweep = self._mode
+        return t
 
     def _buffers(self):
         """Device buffers for one pair, allocated once (shapes are fixed by the pyramid)."""
@@ -167,7 +191,7 @@ class VFSDepthAdjoint:
                 H.call("vfs_upsample", s, H.ptr(lv["flow"]), hh, ww, H.ptr(up["flow"]), ch[0], ch[1], 2,
                        float(p.fScale))
             H.call("vfs_adj_forward_level", s, ctypes.byref(st), ctypes.byref(prm), tp, sp, mp, n_iter, n_warp,
-                   self.checkpoint, H.ptr(b["save"][k]))
+                   self.checkpoint, H.ptr(b["save"][k]), H.ptr(b["ws"]), ctypes.byref(self._tiles(k)))
         lv0 = b["level"][0][0]
         self.flow = lv0["flow"].clone()
         H.call("vfs_zero_masked", s, H.ptr(self.flow), H.ptr(self.v.masks[0]), sizes[0][0], sizes[0][1], 2)
@@ -205,7 +229,7 @@ class VFSDepthAdjoint:
             lv, st = b["level"][k]
             H.call("vfs_adj_reverse_level", s, ctypes.byref(st), ctypes.byref(prm), tp, sp, mp, n_iter, n_warp,
                    self.checkpoint, H.ptr(b["save"][k]), H.ptr(b["ws"]), H.ptr(b["gu"][k]), H.ptr(b["gub"][k]),
-                   H.ptr(b["gf"][k]), H.ptr(b["gref"][k]), H.ptr(b["gpar"][k]), 0 if self.fused else 1)
+                   H.ptr(b["gf"][k]), H.ptr(b["gref"][k]), H.ptr(b["gpar"][k]), ctypes.byref(self._tiles(k)))
             # level start: u = u_bar = fScale up(u_coarse), flow = fScale up(flow_coarse); the
             # coarse level's final u_bar is discarded, so its adjoint stays zero
             if k < nL - 1:

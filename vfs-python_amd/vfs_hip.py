@@ -50,6 +50,13 @@ class VfsPdWs(ctypes.Structure):
     _fields_ = [(n, c_p) for n in ("u_th0", "u_th1", "u_bar2", "v_bar2", "p2", "q2")]
 
 
+class VfsAdjTiles(ctypes.Structure):
+    """struct vfs_adj_tiles (include/vfs_hip.h)."""
+
+    _fields_ = [(n, ctypes.c_int32) for n in ("fwd_tile_x", "fwd_tile_y", "fwd_k", "rec_tile_x", "rec_tile_y",
+                                              "rec_k", "sweep")]
+
+
 class VfsAdjPd(ctypes.Structure):
     """struct vfs_adj_pd (include/vfs_hip.h)."""
 
@@ -111,9 +118,9 @@ _SIGS = {
     "vfs_depth_loss": [c_p] * 6 + [c_i, c_i, c_f, c_f, c_f, c_f, c_f],
     "vfs_reduce_sum": [c_p, c_p, c_p, c_i, c_p, c_i64, c_i],
     "vfs_adj_forward_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
-                              c_p],
+                              c_p, c_p, ctypes.POINTER(VfsAdjTiles)],
     "vfs_adj_reverse_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
-                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i],
+                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, ctypes.POINTER(VfsAdjTiles)],
     "vfs_capture_begin": [ctypes.POINTER(c_p)],
     "vfs_capture_end": [c_p, ctypes.POINTER(c_p)],
     "vfs_plan_launch": [c_p, c_p],
