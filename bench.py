@@ -129,7 +129,7 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
     import autotune
     total_ms, total_launches, total_bytes = 0.0, 0, 0
     per_level = []
-    for (hh, ww), (tx, ty, k) in zip(sizes, tiles):
+    for (hh, ww), (tx, ty, k, nt) in zip(sizes, tiles):
         nblk = -(-n_iter // k)
 
         def ext_px():
@@ -142,8 +142,8 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
 
         ext = ext_px()
         warp_bytes = (41 * ext + 40 * hh * ww) + (nblk - 1) * (73 * ext + 40 * hh * ww)
-        ms = autotune.time_variant(hh, ww, n_iter, (tx, ty, k), reps=n_warp)
-        per_level.append(dict(size=[hh, ww], tile=[tx, ty, k], launches=nblk * n_warp, avg_us=1e3 * ms / nblk,
+        ms = autotune.time_variant(hh, ww, n_iter, (tx, ty, k, nt), reps=n_warp)
+        per_level.append(dict(size=[hh, ww], tile=[tx, ty, k, nt], launches=nblk * n_warp, avg_us=1e3 * ms / nblk,
                               gbs=warp_bytes / (ms * 1e-3) / 1e9))
         total_ms += ms * n_warp
         total_launches += nblk * n_warp
@@ -332,6 +332,33 @@ def run_adjoint(a):
     print(json.dumps(out), flush=True)
 
 
+def run_probe(a):
+    """Launches for the hardware-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
+    three plain copies of 2^26 floats (256 MiB read, 256 MiB written: the counters'
+    scale), then, per pyramid level of the workload, two warps of the blocked
+    primal-dual kernel with the level's autotuned variant (the first warp warms up)."""
+    import torch
+    import autotune
+    import synthetic as S
+    import vfs_hip as H
+    n = 1 << 26
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        H.call("vfs_copy", H.stream(), H.ptr(dst), H.ptr(src), n)
+    torch.cuda.synchronize()
+    w = S.WORKLOADS[a.workload]
+    sizes = [(w["H"], w["W"])]
+    for _ in range(w["nLevel"] - 1):
+        sizes.append((int(sizes[-1][0] / 2.0), int(sizes[-1][1] / 2.0)))
+    for hh, ww in sizes:
+        tile = autotune.tile_for(hh, ww, w["nSolverIters"])
+        autotune.time_variant(hh, ww, w["nSolverIters"], tile, reps=1)
+        log(f"[probe] {hh}x{ww} tile {tile}")
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": a.workload, "copy_floats": n, "sizes": sizes}), flush=True)
+
+
 def cpu_oracle_baseline(ctx, name):
     """float64 NumPy oracle (1 thread) on the full pyramid with 1 of nWarpIters warps per level."""
     from oracle import vfs_oracle as O
@@ -380,6 +407,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--save-flow", default=None, help="write the last step's flow (H, W, 2) to this .npy file")
+    ap.add_argument("--probe", action="store_true",
+                    help="only the launches of the hardware-counter runs (copy kernel + blocked kernel per level)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every pair is independent and this build runs on a single MI355X; "
@@ -394,6 +423,9 @@ def main():
         return
     if a.workload == "adjoint_800":
         run_adjoint(a)
+        return
+    if a.probe:
+        run_probe(a)
         return
     w = S.WORKLOADS[a.workload]
     log(f"[bench] workload {a.workload}: generating the pair and vector fields")

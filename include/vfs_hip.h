@@ -247,7 +247,8 @@ typedef struct vfs_pd_ws {
 int vfs_pd_variants(int* out, int max_variants);
 
 /* n_iter (1 <= n_iter <= k) primal-dual iterations of Python/vfs.py:498-515 in one
- * launch of the (tile_x, tile_y, k) variant: reads the state (u_in, ub_in,
+ * launch of the (tile_x, tile_y, k, threads) variant (threads 0: the first listed
+ * thread count of that tile and k; likewise below): reads the state (u_in, ub_in,
  * vb_in, p_in, q_in; vb/p/q taken as zero when first != 0) and writes it to the
  * out buffers, which must not alias the in buffers.  taus/sigmas/mus: n_iter
  * host values. */
@@ -256,7 +257,7 @@ int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* 
                   const float* vb_in, const float* p_in, const float* q_in, float* u_out,
                   float* ub_out, float* vb_out, float* p_out, float* q_out, int first,
                   const float* taus, const float* sigmas, const float* mus, int n_iter,
-                  int tile_x, int tile_y, int k, const vfs_params* prm);
+                  int tile_x, int tile_y, int k, int threads, const vfs_params* prm);
 
 /* One level of Python/vfs.py:473-521 with the blocked loop: Gaussian + tensor,
  * then per warp: derivatives, ceil(n_iter / k) blocked launches, and one kernel
@@ -266,7 +267,7 @@ int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* 
 int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* ws,
                             const vfs_params* prm, const float* taus, const float* sigmas,
                             const float* mus, int n_iter, int n_warp, int tile_x, int tile_y,
-                            int k);
+                            int k, int threads);
 
 /* ---- batches of pairs (no reference counterpart: the reference solves one pair
  * per VFS object) ---------------------------------------------------------------- */
@@ -279,16 +280,18 @@ size_t vfs_batch_workspace_bytes(int n_pairs);
  * pair i, any sizes) into d_workspace with a synchronous copy.  Call it before
  * recording a graph that uses the level; the tables hold the buffer pointers. */
 int vfs_batch_prepare_level(void* d_workspace, size_t workspace_bytes, const vfs_level* lv,
-                            const vfs_pd_ws* ws, int n_pairs, int tile_x, int tile_y, int k);
+                            const vfs_pd_ws* ws, int n_pairs, int tile_x, int tile_y, int k,
+                            int threads);
 
 /* vfs_solve_level_blocked for every pair of the batch at once: per warp, one
  * batched derivatives launch, ceil(n_iter / k) batched blocked launches over the
  * tiles of all pairs, and one batched median / clamp / next-warp kernel.
- * d_workspace must have been prepared with the same lv, ws, tile_x, tile_y, k. */
+ * d_workspace must have been prepared with the same lv, ws, tile_x, tile_y, k, threads. */
 int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const vfs_level* lv,
                                   const vfs_pd_ws* ws, int n_pairs, const vfs_params* prm,
                                   const float* taus, const float* sigmas, const float* mus,
-                                  int n_iter, int n_warp, int tile_x, int tile_y, int k);
+                                  int n_iter, int n_warp, int tile_x, int tile_y, int k,
+                                  int threads);
 
 /* ---- adjoint (reverse mode) of the solve (no reference counterpart) -------------- */
 
@@ -378,6 +381,14 @@ int vfs_depth_loss(void* stream, float* term, float* gflow, const float* flow, c
 int vfs_reduce_sum(void* stream, double* out, double* partial, int nparts, const float* in, int64_t n,
                    int channels);
 
+/* Residual norm for convergence monitoring (no reference counterpart):
+ * out[0] = sum over pixels with mask != 0 (every pixel when mask is NULL) of
+ * sum_c (a - b)^2 over `channels` interleaved channels, out[1] = the number of
+ * those pixels, both in double and deterministic (wavefront shuffles, then one
+ * block); partial holds 2 * nparts doubles (1 <= nparts <= 65535). */
+int vfs_residual_norm(void* stream, double* out, double* partial, int nparts, const float* a, const float* b,
+                      const uint8_t* mask, int64_t n, int channels);
+
 /* ---- adjoint level driver ---------------------------------------------------------
  * The forward warp loop of one level that records what the reverse needs, and the
  * reverse sweep of one level, each enqueued on one stream without host
@@ -394,18 +405,20 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
 
 /* Kernel choices of the adjoint level driver.  A zero variant selects the
  * per-kernel path.
- *   fwd_tile_x, fwd_tile_y, fwd_k: blocked primal-dual variant (vfs_pd_variants) of
- *     the forward; its launches end at every checkpoint;
- *   rec_tile_x, rec_tile_y, rec_k: blocked variant of the tape recompute; each launch
- *     runs up to rec_k iterations and writes every intermediate state to the tape;
+ *   fwd_tile_x, fwd_tile_y, fwd_k, fwd_threads: blocked primal-dual variant
+ *     (vfs_pd_variants; threads 0 = the first listed) of the forward; its launches
+ *     end at every checkpoint;
+ *   rec_tile_x, rec_tile_y, rec_k, rec_threads: blocked variant of the tape recompute;
+ *     each launch runs up to rec_k iterations and writes every intermediate state to
+ *     the tape;
  *   sweep: reverse primal-dual sweep.  0: one fused kernel per iteration (dual^T of
  *     iteration i with primal^T and threshold^T of iteration i - 1; the dual
  *     messages of a 32 x 8 tile and its one-pixel ring exchanged in LDS); 1: one
  *     kernel per step (vfs_adj_pd_iteration), messages through memory; 11, 12, 14:
  *     fused with tiles of 32 x 8, 32 x 16, 32 x 32 (1, 2, 4 rows per thread). */
 typedef struct vfs_adj_tiles {
-    int32_t fwd_tile_x, fwd_tile_y, fwd_k;
-    int32_t rec_tile_x, rec_tile_y, rec_k;
+    int32_t fwd_tile_x, fwd_tile_y, fwd_k, fwd_threads;
+    int32_t rec_tile_x, rec_tile_y, rec_k, rec_threads;
     int32_t sweep;
 } vfs_adj_tiles;
 

@@ -78,7 +78,8 @@ def _blocked_iters(T, st, n, first, taus, sigmas, mus, prm, variant):
     H.call("vfs_pd_blocks", H.stream(), Hh, Ww, H.ptr(st["mask"]), H.ptr(st["abc"]), H.ptr(st["eta"]),
            H.ptr(st["deriv"]), H.ptr(st["u"]), H.ptr(st["ub"]), H.ptr(st["vb"]), H.ptr(st["p"]), H.ptr(st["q"]),
            H.ptr(out["u"]), H.ptr(out["ub"]), H.ptr(out["vb"]), H.ptr(out["p"]), H.ptr(out["q"]), int(first),
-           t_, s_, m_, n, variant[0], variant[1], variant[2], ctypes.byref(prm))
+           t_, s_, m_, n, variant[0], variant[1], variant[2], variant[3] if len(variant) > 3 else 0,
+           ctypes.byref(prm))
     return out
 
 

@@ -115,3 +115,45 @@ def test_depth_matches_oracle(T, small_case, small_oracle):
     print(f"depth rel diff vs oracle: median {np.median(rel):.3e} p99 {np.percentile(rel, 99):.3e}")
     assert np.median(rel) <= 1e-3
     assert np.all(xyz[~m] == [0, 0, -1])
+
+
+def test_residual_norm_matches_numpy_and_tracks_warps(T):
+    """vfs_residual_norm (masked, multi-channel, deterministic) against NumPy, and the
+    per-warp residuals of the Python-driven solve: one per warp and level, coarse to
+    fine, finite and non-negative."""
+    import torch
+    import synthetic as S
+    import vfs as V
+    import vfs_params as P
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(37, 53, 2)).astype(np.float32)
+    b = rng.normal(size=(37, 53, 2)).astype(np.float32)
+    m = (rng.random((37, 53)) > 0.3).astype(np.uint8) * 255
+    v = V.VFS(P.vfs_params(np.zeros((8, 8), np.uint8)))
+    v.setup_module()
+    ta, tb, tm = (torch.as_tensor(x, device="cuda") for x in (a, b, m))
+    got = float(v.residual_norm(ta, tb, tm))
+    ref = np.sqrt(np.sum(((a - b).astype(np.float64) ** 2)[m > 0]) / np.count_nonzero(m))
+    assert abs(got - ref) <= 1e-6 * ref
+    got1 = float(v.residual_norm(ta[..., 0].contiguous(), tb[..., 0].contiguous()))
+    ref1 = np.sqrt(np.mean((a[..., 0] - b[..., 0]).astype(np.float64) ** 2))
+    assert abs(got1 - ref1) <= 1e-6 * ref1
+    assert float(v.residual_norm(ta, tb, tm)) == got  # deterministic
+    # per-warp tracking in the Python-driven solve
+    n = 64
+    pr = S.make_pair(n, n, 2, t=(0.2, 0.0, 0.0), rot_deg=1.0, n_levels=2)
+    tvf, cvf = O.get_vector_fields(O.get_grid_zero((n, n)), pr["cam"], pr["cam"], pr["R"], pr["t"], 1)
+    prm = P.vfs_params(pr["img_ref"])
+    prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = 2, 2.0, 3, 4
+    prm.Lambda, prm.alpha0, prm.alpha1, prm.beta, prm.gamma = 50.0, 5.0, 1.0, 10.0, 0.5
+    v = V.VFS(prm)
+    v.solver = "python"
+    v.track_residuals = True
+    v.setup_module()
+    v.compute_pyramid_sizes()
+    v.create_mask_pyramid(pr["mask"])
+    v.create_vector_field_pyramid(S.sanitize_field(tvf), cvf.astype(np.float32))
+    v.solve_stereo(pr["img_ref"], pr["img_other"])
+    assert [(lv, w) for lv, w, _ in v.residuals] == [(1, 0), (1, 1), (1, 2), (0, 0), (0, 1), (0, 2)]
+    assert all(np.isfinite(r) and r >= 0 for _, _, r in v.residuals)
+    assert v.residuals[0][2] > 0

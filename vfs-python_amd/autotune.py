@@ -30,22 +30,33 @@ def _load():
         if os.path.exists(TABLE):
             with open(TABLE) as f:
                 for row in json.load(f).get("entries", []):
-                    _table[(int(row["h"]), int(row["w"]), int(row["iters"]))] = tuple(row["best"][:3])
+                    _table[(int(row["h"]), int(row["w"]), int(row["iters"]))] = variant(row["best"])
     return _table
+
+
+def variant(t):
+    """(tile_x, tile_y, k, threads) from a 3- or 4-tuple; threads 0 selects the first
+    compiled thread count of that tile and k."""
+    t = tuple(int(x) for x in t)
+    if len(t) == 3:
+        return t + (0,)
+    if len(t) != 4:
+        raise ValueError(f"a blocked variant is (tile_x, tile_y, k[, threads]), got {t}")
+    return t
 
 
 def default_tile(h, w):
     """Choice before any tuning: larger k where launches dominate (small levels)."""
     n = h * w
     if n >= 250_000:
-        return (32, 32, 4)
+        return (32, 32, 4, 1024)
     if n >= 40_000:
-        return (32, 16, 8)
-    return (16, 16, 8)
+        return (32, 16, 8, 1024)
+    return (16, 16, 8, 1024)
 
 
 def tile_for(h, w, n_iter):
-    """(tile_x, tile_y, k) for a level of h x w pixels and n_iter iterations per warp."""
+    """(tile_x, tile_y, k, threads) for a level of h x w pixels and n_iter iterations per warp."""
     t = _load()
     if (h, w, n_iter) in t:
         return t[(h, w, n_iter)]
@@ -55,8 +66,8 @@ def tile_for(h, w, n_iter):
     return default_tile(h, w)
 
 
-def time_variant(h, w, n_iter, variant, reps=5, seed=0):
-    """Mean time (ms) of one warp's n_iter blocked iterations at h x w with ``variant``."""
+def time_variant(h, w, n_iter, var, reps=5, seed=0):
+    """Mean time (ms) of one warp's n_iter blocked iterations at h x w with the variant ``var``."""
     import numpy as np
     import torch
     import vfs_hip as H
@@ -78,7 +89,7 @@ def time_variant(h, w, n_iter, variant, reps=5, seed=0):
                          p=torch.zeros((h, w, 2), **dev), q=torch.zeros((h, w, 4), **dev)))
     u0 = torch.zeros((h, w), **dev)
     taus, sigmas, mus = H.pd_schedule(n_iter, 1.0)
-    tx, ty, k = variant[:3]
+    tx, ty, k, nt = variant(var)
     s = H.stream()
     fn = H.lib().vfs_pd_blocks
 
@@ -93,7 +104,8 @@ def time_variant(h, w, n_iter, variant, reps=5, seed=0):
                     H.ptr(b["vb"]), H.ptr(b["p"]), H.ptr(b["q"]), 1 if first else 0,
                     np.ascontiguousarray(taus[sl]).ctypes.data_as(ctypes.c_void_p),
                     np.ascontiguousarray(sigmas[sl]).ctypes.data_as(ctypes.c_void_p),
-                    np.ascontiguousarray(mus[sl]).ctypes.data_as(ctypes.c_void_p), kc, tx, ty, k, ctypes.byref(prm))
+                    np.ascontiguousarray(mus[sl]).ctypes.data_as(ctypes.c_void_p), kc, tx, ty, k, nt,
+                    ctypes.byref(prm))
             H.check(rc, "vfs_pd_blocks")
             cur, done, first = 1 - cur, done + kc, False
 
@@ -108,17 +120,22 @@ def time_variant(h, w, n_iter, variant, reps=5, seed=0):
     return e0.elapsed_time(e1) / reps
 
 
-def tune(sizes, n_iter, reps=5, log=print):
+def tune(sizes, n_iter, reps=5, rounds=3, log=print):
+    """Per size, `rounds` passes over every variant (interleaved, so clock or thermal
+    drift affects all variants alike), each timing `reps` warps; a variant's time is
+    its best pass."""
     import vfs_hip as H
     variants = H.pd_variants()
     entries = []
     for (h, w) in sizes:
-        results = []
-        for v in variants:
-            ms = time_variant(h, w, n_iter, v, reps=reps)
-            results.append((ms, v))
+        best_ms = {}
+        for _ in range(rounds):
+            for v in variants:
+                ms = time_variant(h, w, n_iter, v, reps=reps)
+                best_ms[v] = min(ms, best_ms.get(v, float("inf")))
+        results = sorted((ms, v) for v, ms in best_ms.items())
+        for ms, v in results:
             log(f"[autotune] {h}x{w} iters {n_iter} tile {v[0]}x{v[1]} k {v[2]} threads {v[3]}: {ms * 1e3:.1f} us/warp")
-        results.sort()
         best = results[0][1]
         entries.append(dict(h=h, w=w, iters=n_iter, best=list(best), us_per_warp=round(results[0][0] * 1e3, 2),
                             all=[dict(tile=list(v), us=round(ms * 1e3, 2)) for ms, v in results]))
@@ -145,10 +162,11 @@ def main():
     ap.add_argument("--sizes", default="800x800,400x400,200x200,100x100,50x50")
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--rounds", type=int, default=3)
     ap.add_argument("--out", default=TABLE)
     a = ap.parse_args()
     sizes = [tuple(int(x) for x in s.split("x")) for s in a.sizes.split(",")]
-    entries = tune(sizes, a.iters, a.reps, log=lambda *m: print(*m, file=sys.stderr, flush=True))
+    entries = tune(sizes, a.iters, a.reps, a.rounds, log=lambda *m: print(*m, file=sys.stderr, flush=True))
     save(entries, a.out)
     print(json.dumps([dict(h=e["h"], w=e["w"], best=e["best"], us_per_warp=e["us_per_warp"]) for e in entries]))
 

@@ -584,6 +584,45 @@ __global__ void __launch_bounds__(256) k_reduce_final(double* __restrict__ out, 
     }
 }
 
+// Per-block partials of the masked residual: sum of (a - b)^2 over `channels`
+// interleaved channels and the number of mask pixels, in double.
+__global__ void __launch_bounds__(256) k_residual_partial(double* __restrict__ part, const float* __restrict__ a,
+                                                          const float* __restrict__ b,
+                                                          const uint8_t* __restrict__ mask, int64_t n, int channels)
+{
+    __shared__ double sm[2][4];
+    double ss = 0.0, cnt = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (mask != nullptr && mask[i] == 0) continue;
+        float acc = 0.0f;
+        for (int c = 0; c < channels; ++c) {
+            const float d = a[i * channels + c] - b[i * channels + c];
+            acc = fmaf(d, d, acc);
+        }
+        ss += static_cast<double>(acc);
+        cnt += 1.0;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ss += __shfl_down(ss, off, 64);
+        cnt += __shfl_down(cnt, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][threadIdx.x >> 6] = ss;
+        sm[1][threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+            t0 += sm[0][w];
+            t1 += sm[1][w];
+        }
+        part[2 * blockIdx.x] = t0;
+        part[2 * blockIdx.x + 1] = t1;
+    }
+}
+
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace
@@ -744,6 +783,17 @@ int vfs_reduce_sum(void* stream, double* out, double* partial, int nparts, const
     if (out == nullptr || partial == nullptr || in == nullptr || n < 0 || channels < 1 || nparts < 1) return VFS_EINVAL;
     hipLaunchKernelGGL(k_reduce_partial, dim3(nparts), dim3(256), 0, S(stream), partial, in, n, channels);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, S(stream), out, partial, nparts, channels);
+    VFS_LAUNCH_STATUS();
+}
+
+int vfs_residual_norm(void* stream, double* out, double* partial, int nparts, const float* a, const float* b,
+                      const uint8_t* mask, int64_t n, int channels)
+{
+    if (out == nullptr || partial == nullptr || a == nullptr || b == nullptr || n < 0 || channels < 1 ||
+        nparts < 1 || nparts > 65535)
+        return VFS_EINVAL;
+    hipLaunchKernelGGL(k_residual_partial, dim3(nparts), dim3(256), 0, S(stream), partial, a, b, mask, n, channels);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, S(stream), out, partial, nparts, 2);
     VFS_LAUNCH_STATUS();
 }
 

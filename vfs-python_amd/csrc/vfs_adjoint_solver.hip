@@ -359,17 +359,10 @@ __device__ __forceinline__ void put_msg(float* m, int l, const DualMsg& d)
     for (int k = 0; k < 6; ++k) r[3 + k] = d.vm[k];
 }
 
-// Tile of a block.  Workgroups are dispatched to the eight XCDs in turn (block b
-// to XCD b mod 8); tiles are handed out so that each XCD works on one contiguous
-// run of tiles in row-major order, which keeps the halo rows a tile shares with
-// its neighbours in that XCD's L2.
+// Tile of a block, in the XCD-aware order of xcd_item.
 __device__ __forceinline__ void tile_of(int b, int nbx, int nby, int& bx, int& by)
 {
-    const int n = nbx * nby;
-    const int x = b & 7, k = b >> 3;
-    int start = 0;
-    for (int y = 0; y < x; ++y) start += (n - y + 7) >> 3;
-    const int t = start + k;
+    const int t = xcd_item(b, nbx * nby);
     bx = t % nbx;
     by = t / nbx;
 }
@@ -667,7 +660,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
                 VFS_TRY(vfs_pd_blocks(stream, H, W, lv->mask, lv->abc, lv->eta, lv->deriv, first ? lv->u : su[cur],
                                       sub[cur], svb[cur], sp[cur], sq[cur], su[1 - cur], sub[1 - cur], svb[1 - cur],
                                       sp[1 - cur], sq[1 - cur], first ? 1 : 0, taus + done, sigmas + done, mus + done,
-                                      kc, tiles->fwd_tile_x, tiles->fwd_tile_y, fk, prm));
+                                      kc, tiles->fwd_tile_x, tiles->fwd_tile_y, fk, tiles->fwd_threads, prm));
                 cur = 1 - cur;
                 done += kc;
                 first = false;
@@ -814,6 +807,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                 L.tile_x = rtx;
                 L.tile_y = rty;
                 L.k = rk;
+                L.threads = tiles->rec_threads;
                 L.tape = kc > 1 ? state_of(seg, i + 1).p : nullptr;
                 L.tape_stride = state_floats(n);
                 L.tape_off[0] = al(2 * n);

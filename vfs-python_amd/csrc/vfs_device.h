@@ -142,6 +142,18 @@ inline dim3 grid2d(int H, int W)
 
 inline dim3 block2d() { return dim3(VFS_BLOCK_X, VFS_BLOCK_Y, 1); }
 
+// Work item of workgroup b among n.  Workgroups are dispatched to the eight XCDs in
+// turn (b to XCD b mod 8), each XCD with its own L2; items are handed out so that
+// each XCD works through one contiguous run of items, so tiles that share halo
+// rows and cache lines (row-major neighbours) meet in the same L2.
+__device__ __forceinline__ int xcd_item(int b, int n)
+{
+    const int x = b & 7, k = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; ++y) start += (n - y + 7) >> 3;
+    return start + k;
+}
+
 }  // namespace vfs
 
 // Status helpers for the C ABI.

@@ -10,7 +10,7 @@
 namespace vfs {
 
 // One launch of the blocked primal-dual kernel (tile_x x tile_y, k iterations per
-// launch at most) that runs n_iter iterations from the state `in` to the state
+// launch at most, `threads` per workgroup or 0 for the first listed) that runs n_iter iterations from the state `in` to the state
 // `out` and also writes the state after each of the first tape_count iterations
 // into tape + i * tape_stride.  State arrays: p (H,W,2), q (H,W,4), u, u_bar (H,W),
 // v_bar (H,W,2); tape_off = offsets (floats) of q, u, u_bar, v_bar in a tape slot
@@ -22,7 +22,7 @@ struct PdTapeLaunch {
     const float* in[5];  // p, q, u, u_bar, v_bar
     float* out[5];
     const float *taus, *sigmas, *mus;
-    int n_iter, tile_x, tile_y, k;
+    int n_iter, tile_x, tile_y, k, threads;
     float* tape;
     int64_t tape_stride, tape_off[4];
     int tape_count;

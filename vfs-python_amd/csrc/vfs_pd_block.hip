@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(NT) k_pd_block(PDLevel L, int k_count, PDSched
     constexpr int E = (TX + 3 * K) * (TY + 3 * K);
     __shared__ float sA[3 * E];
     __shared__ float sB[6 * E];
-    const int t = blockIdx.x;
+    const int t = xcd_item(blockIdx.x, gridDim.x);
     pd_block_tile<TX, TY, K, NT>(L, t % L.tiles_x, t / L.tiles_x, k_count, sc, cst, sA, sB);
 }
 
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(NT) k_pd_block_batched(const PDLevel* __restri
     constexpr int E = (TX + 3 * K) * (TY + 3 * K);
     __shared__ float sA[3 * E];
     __shared__ float sB[6 * E];
-    const int t = blockIdx.x;
+    const int t = xcd_item(blockIdx.x, gridDim.x);
     int lo = 0, hi = n - 1;  // last i with tile_start[i] <= t
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -498,17 +498,23 @@ struct Variant {
 const Variant kVariants[] = {
     VFS_PD_VARIANT(16, 16, 1, 256),  VFS_PD_VARIANT(16, 16, 2, 512),  VFS_PD_VARIANT(16, 16, 4, 1024),
     VFS_PD_VARIANT(16, 16, 8, 1024), VFS_PD_VARIANT(32, 32, 1, 1024), VFS_PD_VARIANT(32, 32, 2, 1024),
-    VFS_PD_VARIANT(32, 32, 4, 1024), VFS_PD_VARIANT(64, 32, 2, 1024),
+    VFS_PD_VARIANT(32, 32, 4, 1024),
     VFS_PD_VARIANT(64, 16, 2, 1024), VFS_PD_VARIANT(64, 16, 4, 1024),
     VFS_PD_VARIANT(32, 16, 4, 1024), VFS_PD_VARIANT(32, 16, 8, 1024), VFS_PD_VARIANT(8, 8, 8, 512),
     VFS_PD_VARIANT(16, 16, 2, 256),  VFS_PD_VARIANT(32, 32, 2, 256),  VFS_PD_VARIANT(32, 32, 4, 256),
+    // small tiles: more workgroups for the coarse levels
+    VFS_PD_VARIANT(8, 8, 1, 128),    VFS_PD_VARIANT(8, 8, 2, 256),    VFS_PD_VARIANT(8, 8, 4, 512),
+    VFS_PD_VARIANT(16, 8, 2, 256),   VFS_PD_VARIANT(16, 8, 4, 512),   VFS_PD_VARIANT(16, 16, 4, 512),
+    VFS_PD_VARIANT(32, 8, 2, 256),   VFS_PD_VARIANT(32, 8, 4, 512),   VFS_PD_VARIANT(32, 16, 2, 512),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-const Variant* find_variant(int tx, int ty, int k)
+// The variant (tx, ty, k) with nt threads, or with the first listed thread count for nt == 0.
+const Variant* find_variant(int tx, int ty, int k, int nt)
 {
     for (int i = 0; i < kNumVariants; ++i)
-        if (kVariants[i].tx == tx && kVariants[i].ty == ty && kVariants[i].k == k) return &kVariants[i];
+        if (kVariants[i].tx == tx && kVariants[i].ty == ty && kVariants[i].k == k && (nt == 0 || kVariants[i].nt == nt))
+            return &kVariants[i];
     return nullptr;
 }
 
@@ -574,7 +580,7 @@ int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* 
                   const float* deriv, const float* u_in, const float* ub_in, const float* vb_in, const float* p_in,
                   const float* q_in, float* u_out, float* ub_out, float* vb_out, float* p_out, float* q_out,
                   int first, const float* taus, const float* sigmas, const float* mus, int n_iter, int tile_x,
-                  int tile_y, int k, const vfs_params* prm)
+                  int tile_y, int k, int threads, const vfs_params* prm)
 {
     VFS_CHECK_DIMS(H, W);
     const void* ptrs[] = {mask, abc, eta, deriv, u_in, ub_in, u_out, ub_out, vb_out, p_out, q_out, taus, sigmas, mus,
@@ -583,7 +589,7 @@ int vfs_pd_blocks(void* stream, int H, int W, const uint8_t* mask, const float* 
         if (p == nullptr) return VFS_EINVAL;
     if (!first && (vb_in == nullptr || p_in == nullptr || q_in == nullptr)) return VFS_EINVAL;
     if (n_iter < 1 || n_iter > k) return VFS_EINVAL;
-    const Variant* v = find_variant(tile_x, tile_y, k);
+    const Variant* v = find_variant(tile_x, tile_y, k, threads);
     if (v == nullptr) return VFS_EINVAL;
     PDLevel L;
     L.H = H;
@@ -617,7 +623,7 @@ int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm)
     if (prm == nullptr || a.n_iter < 1 || a.n_iter > a.k || a.tape_count < 0 || a.tape_count >= a.n_iter)
         return VFS_EINVAL;
     if (a.tape_count > 0 && a.tape == nullptr) return VFS_EINVAL;
-    const Variant* v = find_variant(a.tile_x, a.tile_y, a.k);
+    const Variant* v = find_variant(a.tile_x, a.tile_y, a.k, a.threads);
     if (v == nullptr) return VFS_EINVAL;
     PDLevel L;
     L.H = a.H;
@@ -654,7 +660,7 @@ extern "C" {
 
 int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* ws, const vfs_params* prm,
                             const float* taus, const float* sigmas, const float* mus, int n_iter, int n_warp,
-                            int tile_x, int tile_y, int k)
+                            int tile_x, int tile_y, int k, int threads)
 {
     if (lv == nullptr || ws == nullptr || prm == nullptr) return VFS_EINVAL;
     VFS_CHECK_DIMS(lv->H, lv->W);
@@ -666,7 +672,7 @@ int vfs_solve_level_blocked(void* stream, const vfs_level* lv, const vfs_pd_ws* 
     if (n_iter < 0 || n_warp < 0 || (n_iter > 0 && (taus == nullptr || sigmas == nullptr || mus == nullptr)))
         return VFS_EINVAL;
     if (prm->median_k != 3 && prm->median_k != 5) return VFS_EINVAL;
-    const Variant* v = find_variant(tile_x, tile_y, k);
+    const Variant* v = find_variant(tile_x, tile_y, k, threads);
     if (v == nullptr || k > KMAX) return VFS_EINVAL;
     hipStream_t s = S(stream);
     const int H = lv->H, W = lv->W;
@@ -783,12 +789,12 @@ size_t vfs_batch_workspace_bytes(int n_pairs)
 }
 
 int vfs_batch_prepare_level(void* d_workspace, size_t workspace_bytes, const vfs_level* lv, const vfs_pd_ws* ws,
-                            int n_pairs, int tile_x, int tile_y, int k)
+                            int n_pairs, int tile_x, int tile_y, int k, int threads)
 {
     if (d_workspace == nullptr || lv == nullptr || ws == nullptr || n_pairs <= 0) return VFS_EINVAL;
     const BatchLayout lay = batch_layout(n_pairs);
     if (workspace_bytes < lay.total) return VFS_EINVAL;
-    if (find_variant(tile_x, tile_y, k) == nullptr) return VFS_EINVAL;
+    if (find_variant(tile_x, tile_y, k, threads) == nullptr) return VFS_EINVAL;
     std::vector<unsigned char> host(lay.total, 0);
     PDLevel* pd = reinterpret_cast<PDLevel*>(host.data() + lay.pd);
     int* pd_start = reinterpret_cast<int*>(host.data() + lay.pd_start);
@@ -857,14 +863,15 @@ int vfs_batch_prepare_level(void* d_workspace, size_t workspace_bytes, const vfs
 
 int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const vfs_level* lv, const vfs_pd_ws* ws,
                                   int n_pairs, const vfs_params* prm, const float* taus, const float* sigmas,
-                                  const float* mus, int n_iter, int n_warp, int tile_x, int tile_y, int k)
+                                  const float* mus, int n_iter, int n_warp, int tile_x, int tile_y, int k,
+                                  int threads)
 {
     if (d_workspace == nullptr || lv == nullptr || ws == nullptr || prm == nullptr || n_pairs <= 0) return VFS_EINVAL;
     if (n_iter < 0 || n_warp < 0 || (n_iter > 0 && (taus == nullptr || sigmas == nullptr || mus == nullptr)))
         return VFS_EINVAL;
     if (prm->median_k != 3 && prm->median_k != 5) return VFS_EINVAL;
     if (n_warp > 0 && n_iter == 0) return VFS_EINVAL;  // the batched median reads this warp's thresholded u
-    const Variant* v = find_variant(tile_x, tile_y, k);
+    const Variant* v = find_variant(tile_x, tile_y, k, threads);
     if (v == nullptr) return VFS_EINVAL;
     hipStream_t s = S(stream);
     const BatchLayout lay = batch_layout(n_pairs);

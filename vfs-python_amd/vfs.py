@@ -22,7 +22,7 @@ attribute ``solver`` selects how it is driven:
   temporally blocked (k iterations per launch on LDS-exchanged register tiles,
   see csrc/vfs_pd_block.hip) and the per-warp kernels fused, recorded or
   enqueued per call.  The tile shape and k per pyramid level come from
-  ``pd_tiles`` (a dict {level: (tile_x, tile_y, k)}) or from the autotuned
+  ``pd_tiles`` (a dict {level: (tile_x, tile_y, k[, threads])}) or from the autotuned
   table (autotune.py).
 """
 
@@ -61,6 +61,10 @@ class VFS:
         self.param = param
         self.gpu_module = None
         self.solver = "graph"
+        # solver="python" only: per warp, the RMS over the mask of (thresholded u -
+        # warp-start u), as (level, warp, rms) in self.residuals after a solve
+        self.track_residuals = False
+        self.residuals = []
         self.pd_tiles = None
         self.median_filter_kernel_size = 5
         self._plan = None
@@ -344,12 +348,12 @@ class VFS:
                 self.calibration.data_ptr(), id(b), self.solver, tuple(self._tiles()))
 
     def _tiles(self):
-        """(tile_x, tile_y, k) of the blocked primal-dual loop for every pyramid level."""
+        """(tile_x, tile_y, k, threads) of the blocked primal-dual loop for every pyramid level."""
         import autotune
         out = []
         for lvl, (h, w) in enumerate(self.sizes):
             if self.pd_tiles is not None and lvl in self.pd_tiles:
-                out.append(tuple(self.pd_tiles[lvl]))
+                out.append(autotune.variant(self.pd_tiles[lvl]))
             else:
                 out.append(autotune.tile_for(h, w, int(self.param.nSolverIters)))
         return out
@@ -437,9 +441,9 @@ class VFS:
                 ws = H.VfsPdWs()
                 for name, t in b["ws"][k - 1].items():
                     setattr(ws, name, t.data_ptr())
-                tx, ty, kk = tiles[k - 1]
+                tx, ty, kk, nt = tiles[k - 1]
                 H.call("vfs_solve_level_blocked", s, ctypes.byref(lv), ctypes.byref(ws), ctypes.byref(prm), *sched,
-                       int(p.nSolverIters), int(p.nWarpIters), tx, ty, kk)
+                       int(p.nSolverIters), int(p.nWarpIters), tx, ty, kk, nt)
             else:
                 H.call("vfs_solve_level_plain", s, ctypes.byref(lv), ctypes.byref(prm), *sched,
                        int(p.nSolverIters), int(p.nWarpIters))
@@ -450,6 +454,7 @@ class VFS:
         self.create_image_pyramid(img_ref, img_other)
         taus, sigmas, mus = H.pd_schedule(int(self.param.nSolverIters), self.param.timeStepLambda)
         warpUV_ref = u = u_ = None
+        res = []
         for s, k in zip(self.sizes[::-1], range(len(self.sizes), 0, -1)):
             s2 = (*s, 2)
             s4 = (*s, 4)
@@ -472,6 +477,8 @@ class VFS:
                     u = self.l1_thresholding(taus[ks], tp, u, u_, derivatives, eta, self.masks_to[k - 1])
                     u_, v_ = self.update_primal_variables(taus[ks], mus[ks], p, q, u, u_, v_, abc, eta,
                                                           self.masks_to[k - 1])
+                if self.track_residuals:
+                    res.append((k - 1, kw, self.residual_norm(u, u_before_solver, self.masks[k - 1])))
                 u_filtered = self.median_filter(u, self.masks_to[k - 1])
                 warpUV_ref, u = self.compute_optical_flow_with_clamp(warpUV_ref, u_before_solver, u_filtered,
                                                                      translation_vector_ref, self.masks_to[k - 1])
@@ -482,7 +489,27 @@ class VFS:
                 warpUV_ref = self.upsampling_vector_field(warpUV_ref, ss)
         warpUV_ref[self.masks[0] == 0, :] = 0
         self.warpUV = warpUV_ref
+        self.residuals = [(lv, w, float(r)) for lv, w, r in res]
         return warpUV_ref
+
+    def residual_norm(self, a, b, mask=None):
+        """sqrt(mean over mask pixels of |a - b|^2) as a 0-d float64 device tensor (no host
+        synchronisation); a, b: (H, W[, C]) float32 on the device, mask (H, W) uint8 or None."""
+        assert a.shape == b.shape and a.dtype == torch.float32 and b.dtype == torch.float32
+        a, b = a.contiguous(), b.contiguous()
+        n = a.shape[0] * a.shape[1]
+        ch = 1 if a.dim() == 2 else a.shape[2]
+        nparts = max(1, min(1024, (n + 255) // 256))
+        part = torch.empty(2 * nparts, dtype=torch.float64, device=_DEV)
+        out = torch.empty(2, dtype=torch.float64, device=_DEV)
+        m = None
+        if mask is not None:
+            m = mask.tensor if isinstance(mask, texture.SampledImage) else mask
+            assert m.dtype == torch.uint8 and tuple(m.shape) == tuple(a.shape[:2])
+            m = m.contiguous()
+        H.call("vfs_residual_norm", H.stream(), H.ptr(out), H.ptr(part), nparts, H.ptr(a), H.ptr(b),
+               H.ptr(m) if m is not None else None, n, ch)
+        return torch.sqrt(out[0] / torch.clamp(out[1], min=1.0))
 
     # -- triangulation (Python/vfs.py:535-559) -------------------------------------------
     def triangulate_flow(self, camera, translation):

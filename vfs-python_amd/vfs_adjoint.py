@@ -60,7 +60,7 @@ class VFSDepthAdjoint:
         # forward: blocked primal-dual launches (variant per level from the autotune table)
         # or the per-kernel steps; tape recompute: blocked launches that also write
         # their intermediate states, with the forward's variant ("auto") or a given
-        # (tile_x, tile_y, k), or the per-kernel steps (None)
+        # (tile_x, tile_y, k[, threads]), or the per-kernel steps (None)
         self.blocked = bool(blocked)
         self.rec_tile = rec_tile if rec_tile in (None, "auto") else tuple(int(x) for x in rec_tile)
         if self.checkpoint < 1:
@@ -99,9 +99,10 @@ class VFSDepthAdjoint:
         hh, ww = self.sizes[k]
         best = autotune.tile_for(hh, ww, int(self.param.nSolverIters))
         if self.blocked:
-            t.fwd_tile_x, t.fwd_tile_y, t.fwd_k = best
+            t.fwd_tile_x, t.fwd_tile_y, t.fwd_k, t.fwd_threads = best
         if self.rec_tile is not None:
-            t.rec_tile_x, t.rec_tile_y, t.rec_k = best if self.rec_tile == "auto" else self.rec_tile
+            rec = best if self.rec_tile == "auto" else autotune.variant(self.rec_tile)
+            t.rec_tile_x, t.rec_tile_y, t.rec_k, t.rec_threads = rec
         t.sweep = self._mode
         return t
 

@@ -52,12 +52,12 @@ class VFSBatch:
         self._plan_key = None
 
     def _tiles(self):
-        """One (tile_x, tile_y, k) per pyramid level for all pairs: the tuned choice for the largest pair."""
+        """One (tile_x, tile_y, k, threads) per pyramid level for all pairs: the tuned choice for the largest pair."""
         n_level = len(self.vfs[0].sizes)
         out = []
         for lvl in range(n_level):
             if self.pd_tiles is not None and lvl in self.pd_tiles:
-                out.append(tuple(self.pd_tiles[lvl]))
+                out.append(autotune.variant(self.pd_tiles[lvl]))
                 continue
             h, w = max((v.sizes[lvl] for v in self.vfs), key=lambda s: s[0] * s[1])
             out.append(autotune.tile_for(h, w, int(self.param.nSolverIters)))
@@ -132,10 +132,10 @@ class VFSBatch:
         for lvl in range(len(self.vfs[0].sizes)):
             ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
             levels, wss = self._level_structs(bufs, lvl)
-            tx, ty, k = tiles[lvl]
+            tx, ty, k, nt = tiles[lvl]
             torch.cuda.synchronize()
             H.call("vfs_batch_prepare_level", H.ptr(ws), nbytes, ctypes.cast(levels, ctypes.c_void_p),
-                   ctypes.cast(wss, ctypes.c_void_p), n, tx, ty, k)
+                   ctypes.cast(wss, ctypes.c_void_p), n, tx, ty, k, nt)
             self._ws.append(ws)
             self._structs.append((levels, wss))
 
@@ -153,9 +153,9 @@ class VFSBatch:
             for v, b in zip(self.vfs, bufs):
                 v._enqueue_level_init(s, b, k)
             levels, wss = self._structs[k - 1]
-            tx, ty, kk = tiles[k - 1]
+            tx, ty, kk, nt = tiles[k - 1]
             H.call("vfs_solve_level_blocked_batch", s, H.ptr(self._ws[k - 1]), ctypes.cast(levels, ctypes.c_void_p),
                    ctypes.cast(wss, ctypes.c_void_p), n, ctypes.byref(prm), *sched, int(p.nSolverIters),
-                   int(p.nWarpIters), tx, ty, kk)
+                   int(p.nWarpIters), tx, ty, kk, nt)
         for v, b in zip(self.vfs, bufs):
             v._enqueue_result(s, b)

@@ -53,8 +53,8 @@ class VfsPdWs(ctypes.Structure):
 class VfsAdjTiles(ctypes.Structure):
     """struct vfs_adj_tiles (include/vfs_hip.h)."""
 
-    _fields_ = [(n, ctypes.c_int32) for n in ("fwd_tile_x", "fwd_tile_y", "fwd_k", "rec_tile_x", "rec_tile_y",
-                                              "rec_k", "sweep")]
+    _fields_ = [(n, ctypes.c_int32) for n in ("fwd_tile_x", "fwd_tile_y", "fwd_k", "fwd_threads", "rec_tile_x",
+                                              "rec_tile_y", "rec_k", "rec_threads", "sweep")]
 
 
 class VfsAdjPd(ctypes.Structure):
@@ -99,12 +99,13 @@ _SIGS = {
     "vfs_plan_create_level_plain": [ctypes.POINTER(c_p), ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams),
                                     c_p, c_p, c_p, c_i, c_i],
     "vfs_pd_variants": [c_p, c_i],
-    "vfs_pd_blocks": [c_p, c_i, c_i] + [c_p] * 14 + [c_i, c_p, c_p, c_p, c_i, c_i, c_i, c_i, ctypes.POINTER(VfsParams)],
+    "vfs_pd_blocks": [c_p, c_i, c_i] + [c_p] * 14 + [c_i, c_p, c_p, c_p, c_i, c_i, c_i, c_i, c_i,
+                                                     ctypes.POINTER(VfsParams)],
     "vfs_solve_level_blocked": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsPdWs), ctypes.POINTER(VfsParams),
-                                c_p, c_p, c_p, c_i, c_i, c_i, c_i, c_i],
-    "vfs_batch_prepare_level": [c_p, ctypes.c_size_t, c_p, c_p, c_i, c_i, c_i, c_i],
+                                c_p, c_p, c_p, c_i, c_i, c_i, c_i, c_i, c_i],
+    "vfs_batch_prepare_level": [c_p, ctypes.c_size_t, c_p, c_p, c_i, c_i, c_i, c_i, c_i],
     "vfs_solve_level_blocked_batch": [c_p, c_p, c_p, c_p, c_i, ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i,
-                                      c_i, c_i, c_i],
+                                      c_i, c_i, c_i, c_i],
     "vfs_adj_pd_iteration": [c_p, ctypes.POINTER(VfsAdjPd), ctypes.POINTER(VfsParams), c_f, c_f, c_f, c_i],
     "vfs_adj_pd_finish": [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i, c_i],
     "vfs_adj_clamp": [c_p] * 10 + [c_i, c_i, c_f, c_i],
@@ -117,6 +118,7 @@ _SIGS = {
     "vfs_adj_upsample": [c_p, c_p, c_i, c_i, c_p, c_i, c_i, c_i, c_f],
     "vfs_depth_loss": [c_p] * 6 + [c_i, c_i, c_f, c_f, c_f, c_f, c_f],
     "vfs_reduce_sum": [c_p, c_p, c_p, c_i, c_p, c_i64, c_i],
+    "vfs_residual_norm": [c_p, c_p, c_p, c_i, c_p, c_p, c_p, c_i64, c_i],
     "vfs_adj_forward_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
                               c_p, c_p, ctypes.POINTER(VfsAdjTiles)],
     "vfs_adj_reverse_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
