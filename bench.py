@@ -116,14 +116,15 @@ def dual_kernel_roofline(sizes, n_warp, n_iter, prm):
                 achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
 
 
-def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
+def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter, levels=None):
     """Average duration and algorithmic bandwidth of the blocked primal-dual kernel over one solve's launches.
 
     Bytes per launch (compulsory traffic of the formulation): every pixel of the
     tile extended by 2k (west, north) and k (east, south), clipped to the image,
     reads 73 B (u, u_bar, v_bar, p, q = 40 B; abc, eta, deriv = 32 B; mask 1 B;
     the warp's first launch reads 41 B, its v_bar, p, q being zero), and every
-    tile pixel writes 40 B.
+    tile pixel writes 40 B.  Each level's warp is recorded into a HIP graph and replayed
+    (as in the solve), on the solved pair's own level data when `levels` is given.
     """
     import torch
     import autotune
@@ -142,7 +143,8 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
 
         ext = ext_px()
         warp_bytes = (41 * ext + 40 * hh * ww) + (nblk - 1) * (73 * ext + 40 * hh * ww)
-        ms = autotune.time_variant(hh, ww, n_iter, (tx, ty, k, nt), reps=n_warp)
+        ms = autotune.time_variant(hh, ww, n_iter, (tx, ty, k, nt), reps=n_warp,
+                                   level=None if levels is None else levels[len(per_level)])
         per_level.append(dict(size=[hh, ww], tile=[tx, ty, k, nt], launches=nblk * n_warp, avg_us=1e3 * ms / nblk,
                               gbs=warp_bytes / (ms * 1e-3) / 1e9))
         total_ms += ms * n_warp
@@ -152,6 +154,26 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter):
     bytes_per_launch = total_bytes / total_launches
     return dict(avg_us=avg_s * 1e6, bytes_per_launch=bytes_per_launch,
                 achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
+
+
+def measured_traffic(per_level):
+    """HBM bytes per k_pd_block launch, averaged over the launches of one solve like
+    `achieved`, from the FETCH_SIZE / WRITE_SIZE runs recorded in
+    profiles/pmc_traffic.json (each counter scaled by its reading for a plain copy of
+    known size).  None unless every level ran the variant that was measured."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = {(tuple(e["size"]), tuple(e["tile"])): e for e in json.load(f)["levels"]}
+    tot_b, tot_n = 0.0, 0
+    for lv in per_level:
+        e = rec.get((tuple(lv["size"]), tuple(lv["tile"])))
+        if e is None:
+            return None
+        tot_b += lv["launches"] * (e["fetch_bytes"] + e["write_bytes"])
+        tot_n += lv["launches"]
+    return round(tot_b / tot_n)
 
 
 def run_batch(a):
@@ -476,14 +498,18 @@ def main():
     if not a.no_roofline:
         if a.solver in ("blocked", "blocked_eager"):
             log("[bench] blocked-kernel roofline pass (HIP events)")
-            r = blocked_kernel_roofline(sizes, v._tiles(), w["nWarpIters"], w["nSolverIters"])
+            b = v._buffers()
+            lv_data = [dict(mask=v.masks[i], abc=b["levels"][i]["abc"], eta=b["levels"][i]["eta"],
+                            deriv=b["levels"][i]["deriv"]) for i in range(len(sizes))]
+            r = blocked_kernel_roofline(sizes, v._tiles(), w["nWarpIters"], w["nSolverIters"], lv_data)
             kname = "k_pd_block"
         else:
             log("[bench] dual-kernel roofline pass (HIP events)")
             r = dual_kernel_roofline(sizes, w["nWarpIters"], w["nSolverIters"], v.param)
             kname = "k_update_dual"
+        traffic = measured_traffic(r["per_level"]) if kname == "k_pd_block" else None
         roof = dict(bound="hbm", kernel=kname, achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                     avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
                     per_level=r["per_level"])
 

@@ -66,8 +66,14 @@ def tile_for(h, w, n_iter):
     return default_tile(h, w)
 
 
-def time_variant(h, w, n_iter, var, reps=5, seed=0):
-    """Mean time (ms) of one warp's n_iter blocked iterations at h x w with the variant ``var``."""
+def time_variant(h, w, n_iter, var, reps=5, seed=0, graph=True, level=None):
+    """Mean time (ms) of one warp's n_iter blocked iterations at h x w with the variant ``var``.
+
+    With ``graph`` (as the solver runs them) the warp's launches are recorded once into
+    a HIP graph and the graph is replayed between events on the launch stream, so the
+    time is that of the kernels, not of issuing them from Python.  ``level`` (a dict of
+    mask (h, w) uint8, abc, eta (h, w, 3), deriv (h, w, 2) device tensors) replaces the
+    seeded random level data, e.g. with a solved pair's own."""
     import numpy as np
     import torch
     import vfs_hip as H
@@ -79,10 +85,13 @@ def time_variant(h, w, n_iter, var, reps=5, seed=0):
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     dev = dict(device="cuda")
-    mask = (torch.rand((h, w), generator=g, **dev) > 0.1).to(torch.uint8)
-    abc = torch.rand((h, w, 3), generator=g, **dev)
-    eta = torch.rand((h, w, 3), generator=g, **dev) * 20 + 1
-    deriv = torch.randn((h, w, 2), generator=g, **dev) * 0.1
+    if level is not None:
+        mask, abc, eta, deriv = (level[n].contiguous() for n in ("mask", "abc", "eta", "deriv"))
+    else:
+        mask = (torch.rand((h, w), generator=g, **dev) > 0.1).to(torch.uint8)
+        abc = torch.rand((h, w, 3), generator=g, **dev)
+        eta = torch.rand((h, w, 3), generator=g, **dev) * 20 + 1
+        deriv = torch.randn((h, w, 2), generator=g, **dev) * 0.1
     sets = []
     for _ in range(2):
         sets.append(dict(u=torch.zeros((h, w), **dev), ub=torch.zeros((h, w), **dev), vb=torch.zeros((h, w, 2), **dev),
@@ -93,7 +102,7 @@ def time_variant(h, w, n_iter, var, reps=5, seed=0):
     s = H.stream()
     fn = H.lib().vfs_pd_blocks
 
-    def run_warp():
+    def run_warp(s=s):
         cur, done, first = 0, 0, True
         while done < n_iter:
             kc = min(k, n_iter - done)
@@ -109,15 +118,31 @@ def time_variant(h, w, n_iter, var, reps=5, seed=0):
             H.check(rc, "vfs_pd_blocks")
             cur, done, first = 1 - cur, done + kc, False
 
-    run_warp()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        run_warp()
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    plan = None
+    if graph:
+        cs = ctypes.c_void_p()
+        H.call("vfs_capture_begin", ctypes.byref(cs))
+        run_warp(cs)
+        plan = ctypes.c_void_p()
+        H.call("vfs_capture_end", cs, ctypes.byref(plan))
+
+        def step():
+            H.call("vfs_plan_launch", plan, s)
+    else:
+        step = run_warp
+    try:
+        step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+    finally:
+        if plan is not None:
+            H.lib().vfs_plan_destroy(plan)
 
 
 def tune(sizes, n_iter, reps=5, rounds=3, log=print):
