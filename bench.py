@@ -156,6 +156,43 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter, levels=None):
                 achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
 
 
+def run_fit(a):
+    """The scaled parameter fit (fit.FIT_DEMO): 16 pairs at 512x512, 12 Adam steps on the
+    5 solver parameters from the reference's defaults through the adjoint (truncated
+    gradient); loss and median relative depth error of 8 held-out pairs before and after.  The timed region is the fit itself (--steps / --warmup do
+    not apply)."""
+    import fit
+    r = fit.run_spec(dict(fit.FIT_DEMO), log=log)
+    out = {
+        "metric": METRIC,
+        "value": round(r["pair_gradients_per_s"], 3),
+        "unit": "pair gradients/s",
+        "n_gpus": 1,
+        "steps": r["spec"]["steps"],
+        "warmup": 0,
+        "ms_per_step": round(1e3 * r["fit_seconds"] / r["spec"]["steps"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "fit_512", "size": list(r["spec"]["size"]), "pairs": r["spec"]["pairs"],
+                   "chunk": r["spec"]["chunk"], "adam_steps": r["spec"]["steps"], "held_out": r["spec"]["held_out"],
+                   "schedule": fit.SCHEDULE},
+        "fit_seconds": round(r["fit_seconds"], 3),
+        "start": r["start"],
+        "fitted": {k: round(v, 5) for k, v in r["fitted"].items()},
+        "gradient": "truncated (last warp of the finest level)" if r["spec"]["reverse_warps"] else "exact",
+        "held_out_loss": {"start": round(r["held_out_loss_start"], 5), "fitted": round(r["held_out_loss_fitted"], 5)},
+        "held_out_median_rel_depth_err": {"start": round(r["held_out_median_rel_depth_err_start"], 5),
+                                          "fitted": round(r["held_out_median_rel_depth_err_fitted"], 5)},
+        "train_loss": [round(x, 5) for x in r["train_loss"]],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(per_level):
     """HBM bytes per k_pd_block launch, averaged over the launches of one solve like
     `achieved`, from the FETCH_SIZE / WRITE_SIZE runs recorded in
@@ -421,7 +458,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain_full_800",
-                    choices=["plain_full_800", "plain_small_256", "batched_64", "adjoint_800"])
+                    choices=["plain_full_800", "plain_small_256", "batched_64", "adjoint_800", "fit_512"])
     ap.add_argument("--pairs", type=int, default=64, help="batched_64: number of pairs (first n of the 64)")
     ap.add_argument("--solver", default="blocked", choices=["graph", "eager", "blocked", "blocked_eager"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain per-kernel path beside it")
@@ -445,6 +482,9 @@ def main():
         return
     if a.workload == "adjoint_800":
         run_adjoint(a)
+        return
+    if a.workload == "fit_512":
+        run_fit(a)
         return
     if a.probe:
         run_probe(a)

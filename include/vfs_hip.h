@@ -403,8 +403,8 @@ int vfs_residual_norm(void* stream, double* out, double* partial, int nparts, co
 size_t vfs_adj_save_bytes(int H, int W, int n_warp, int n_iter, int ckpt);
 size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
 
-/* Kernel choices of the adjoint level driver.  A zero variant selects the
- * per-kernel path.
+/* Options of the adjoint level driver: kernel choices (a zero variant selects the
+ * per-kernel path) and the depth of the reverse sweep.
  *   fwd_tile_x, fwd_tile_y, fwd_k, fwd_threads: blocked primal-dual variant
  *     (vfs_pd_variants; threads 0 = the first listed) of the forward; its launches
  *     end at every checkpoint;
@@ -415,23 +415,27 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
  *     iteration i with primal^T and threshold^T of iteration i - 1; the dual
  *     messages of a 32 x 8 tile and its one-pixel ring exchanged in LDS); 1: one
  *     kernel per step (vfs_adj_pd_iteration), messages through memory; 11, 12, 14:
- *     fused with tiles of 32 x 8, 32 x 16, 32 x 32 (1, 2, 4 rows per thread). */
-typedef struct vfs_adj_tiles {
+ *     fused with tiles of 32 x 8, 32 x 16, 32 x 32 (1, 2, 4 rows per thread);
+ *   reverse_warps: 0 = the exact reverse of every warp; r > 0 = only the last r warps
+ *     of the level are reversed and their start state is held fixed (the adjoints of
+ *     the level's initial u, u_bar, flow are returned as zero): a truncated gradient. */
+typedef struct vfs_adj_opts {
     int32_t fwd_tile_x, fwd_tile_y, fwd_k, fwd_threads;
     int32_t rec_tile_x, rec_tile_y, rec_k, rec_threads;
     int32_t sweep;
-} vfs_adj_tiles;
+    int32_t reverse_warps;
+} vfs_adj_opts;
 
 /* Same results as vfs_solve_level_plain (the per-kernel warp loop of
  * Python/vfs.py:473-521; the blocked variants give the same values), plus the
  * save area.  workspace (vfs_adj_workspace_bytes) holds the second state set of
- * the blocked path (unused by the per-kernel path); tiles may be NULL (per-kernel). */
+ * the blocked path (unused by the per-kernel path); opts may be NULL (per-kernel). */
 int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
                           const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save,
-                          float* workspace, const vfs_adj_tiles* tiles);
+                          float* workspace, const vfs_adj_opts* tiles);
 
-/* Reverse of vfs_adj_forward_level (tiles: see vfs_adj_tiles; NULL = per-kernel
- * recompute and sweep 0).  lv must still hold the forward's ref_smoothed,
+/* Reverse of vfs_adj_forward_level (opts: see vfs_adj_opts; NULL = per-kernel
+ * recompute, sweep 0, every warp).  lv must still hold the forward's ref_smoothed,
  * abc and eta; its tv, warped, deriv, p, q, tp, v_bar, u0, uf are overwritten.
  * In: gu, gub (H,W) and gf (H,W,2) are the adjoints of the level's final u, u_bar
  * and flow; on return they are the adjoints of the level's initial u, u_bar and
@@ -441,7 +445,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
 int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
                           const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt,
                           const float* save, float* workspace, float* gu, float* gub, float* gf, float* gref,
-                          float* gpar, const vfs_adj_tiles* tiles);
+                          float* gpar, const vfs_adj_opts* tiles);
 
 /* Library version string (static storage). */
 const char* vfs_version(void);

@@ -611,7 +611,7 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt)
 
 int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
                           const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, float* save,
-                          float* workspace, const vfs_adj_tiles* tiles)
+                          float* workspace, const vfs_adj_opts* tiles)
 {
     VFS_TRY(check_args(lv, prm, taus, sigmas, mus, n_iter, n_warp, ckpt));
     if (save == nullptr && n_warp > 0) return VFS_EINVAL;
@@ -715,7 +715,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
 int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* prm, const float* taus,
                           const float* sigmas, const float* mus, int n_iter, int n_warp, int ckpt, const float* save,
                           float* workspace, float* gu, float* gub, float* gf, float* gref, float* gpar,
-                          const vfs_adj_tiles* tiles)
+                          const vfs_adj_opts* tiles)
 {
     const int mode = tiles ? tiles->sweep : 0;
     const int rtx = tiles ? tiles->rec_tile_x : 0, rty = tiles ? tiles->rec_tile_y : 0;
@@ -763,7 +763,10 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     fa.nbx = nbx;
     fa.nby = fnby;
     const bool fused = mode != 1;
-    for (int w = n_warp - 1; w >= 0; --w) {
+    const int rw = tiles ? tiles->reverse_warps : 0;
+    if (rw < 0) return VFS_EINVAL;
+    const int w_end = (rw > 0 && rw < n_warp) ? n_warp - rw : 0;
+    for (int w = n_warp - 1; w >= w_end; --w) {
         const WarpSave ws = warp_save_at(sv, n, n_iter, ckpt, w);
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, ws.flow, H, W, lv->tvf, H, W));
         VFS_TRY(vfs_warp_image(stream, lv->warped, ws.flow, H, W, lv->other, H, W));
@@ -933,6 +936,12 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         VFS_TRY(vfs_fill(stream, A.gow, 0.0f, n));
         VFS_TRY(vfs_adj_derivatives(stream, gref, A.gow, A.gderiv, lv->tvf, H, W));
         VFS_TRY(vfs_adj_warp(stream, gf, A.gow, A.gtv, ws.flow, lv->other, lv->tvf, H, W));
+    }
+    if (rw > 0) {
+        // truncated: the level's start state is held fixed
+        VFS_TRY(vfs_fill(stream, gu, 0.0f, n));
+        VFS_TRY(vfs_fill(stream, gub, 0.0f, n));
+        VFS_TRY(vfs_fill(stream, gf, 0.0f, 2 * n));
     }
     if (n_warp > 0 && nck > 0 && fused) {
         hipLaunchKernelGGL(k_fold_gpart, dim3(nb1(fblocks)), dim3(256), 0, s, gpar, A.gpart, W, nbx, tile_y,

@@ -47,7 +47,7 @@ def _e(shape, ch=None):
 
 class VFSDepthAdjoint:
     def __init__(self, param, mask, tvf, cvf, cam, t, z_gt, checkpoint=25, fused=True, rows=None, blocked=True,
-                 rec_tile="auto"):
+                 rec_tile="auto", workspace=None, reverse_warps=0):
         import vfs as V
         self.param = copy.copy(param)
         self.checkpoint = int(checkpoint)
@@ -63,6 +63,9 @@ class VFSDepthAdjoint:
         # (tile_x, tile_y, k[, threads]), or the per-kernel steps (None)
         self.blocked = bool(blocked)
         self.rec_tile = rec_tile if rec_tile in (None, "auto") else tuple(int(x) for x in rec_tile)
+        # 0: exact gradient; r > 0: reverse only the last r warps of each level with their
+        # start state held fixed (so only level 0 reaches the loss)
+        self.reverse_warps = int(reverse_warps)
         if self.checkpoint < 1:
             raise ValueError("checkpoint must be >= 1")
         self.v = V.VFS(self.param)
@@ -77,6 +80,9 @@ class VFSDepthAdjoint:
         self.s = None
         self._b = None
         self._done_forward = False
+        # optional shared workspace: a dict that instances solved one after another on the
+        # same stream may share ({"ws": uint8 device tensor}, grown on demand)
+        self._shared_ws = workspace
 
     # -- helpers ------------------------------------------------------------------------------
     def _prm(self):
@@ -95,7 +101,7 @@ class VFSDepthAdjoint:
 
     def _tiles(self, k):
         import autotune
-        t = H.VfsAdjTiles()
+        t = H.VfsAdjOpts()
         hh, ww = self.sizes[k]
         best = autotune.tile_for(hh, ww, int(self.param.nSolverIters))
         if self.blocked:
@@ -104,6 +110,7 @@ class VFSDepthAdjoint:
             rec = best if self.rec_tile == "auto" else autotune.variant(self.rec_tile)
             t.rec_tile_x, t.rec_tile_y, t.rec_k, t.rec_threads = rec
         t.sweep = self._mode
+        t.reverse_warps = self.reverse_warps
         return t
 
     def _buffers(self):
@@ -138,9 +145,18 @@ class VFSDepthAdjoint:
             b["gref"].append(_z(sz))
             b["gpar"].append(_z(sz, 5))
         ws = max(L.vfs_adj_workspace_bytes(sz[0], sz[1], self.checkpoint) for sz in sizes)
-        b["ws"] = torch.empty(ws, dtype=torch.uint8, device="cuda")
+        if self._shared_ws is not None:
+            cur = self._shared_ws.get("ws")
+            if cur is None or cur.numel() < ws:
+                self._shared_ws["ws"] = torch.empty(ws, dtype=torch.uint8, device="cuda")
+            b["ws"] = None
+        else:
+            b["ws"] = torch.empty(ws, dtype=torch.uint8, device="cuda")
         self._b = b
         return b
+
+    def _ws(self):
+        return self._shared_ws["ws"] if self._shared_ws is not None else self._b["ws"]
 
     def _reduce(self, x, channels):
         """Device (channels,) float64 sums of an interleaved float32 array."""
@@ -192,7 +208,7 @@ class VFSDepthAdjoint:
                 H.call("vfs_upsample", s, H.ptr(lv["flow"]), hh, ww, H.ptr(up["flow"]), ch[0], ch[1], 2,
                        float(p.fScale))
             H.call("vfs_adj_forward_level", s, ctypes.byref(st), ctypes.byref(prm), tp, sp, mp, n_iter, n_warp,
-                   self.checkpoint, H.ptr(b["save"][k]), H.ptr(b["ws"]), ctypes.byref(self._tiles(k)))
+                   self.checkpoint, H.ptr(b["save"][k]), H.ptr(self._ws()), ctypes.byref(self._tiles(k)))
         lv0 = b["level"][0][0]
         self.flow = lv0["flow"].clone()
         H.call("vfs_zero_masked", s, H.ptr(self.flow), H.ptr(self.v.masks[0]), sizes[0][0], sizes[0][1], 2)
@@ -225,15 +241,18 @@ class VFSDepthAdjoint:
                 b[name][k].zero_()
         torch.div(self.gflow, self.n_valid, out=b["gf"][0])
         tp, sp, mp = taus.ctypes.data, sigmas.ctypes.data, mus.ctypes.data
-        for k in range(nL):
+        # truncated: the finest level's start state is held fixed, so nothing reaches the
+        # coarser levels (their reverse would run on zero adjoints)
+        n_rev = 1 if self.reverse_warps > 0 else nL
+        for k in range(n_rev):
             hh, ww = sizes[k]
             lv, st = b["level"][k]
             H.call("vfs_adj_reverse_level", s, ctypes.byref(st), ctypes.byref(prm), tp, sp, mp, n_iter, n_warp,
-                   self.checkpoint, H.ptr(b["save"][k]), H.ptr(b["ws"]), H.ptr(b["gu"][k]), H.ptr(b["gub"][k]),
+                   self.checkpoint, H.ptr(b["save"][k]), H.ptr(self._ws()), H.ptr(b["gu"][k]), H.ptr(b["gub"][k]),
                    H.ptr(b["gf"][k]), H.ptr(b["gref"][k]), H.ptr(b["gpar"][k]), ctypes.byref(self._tiles(k)))
             # level start: u = u_bar = fScale up(u_coarse), flow = fScale up(flow_coarse); the
             # coarse level's final u_bar is discarded, so its adjoint stays zero
-            if k < nL - 1:
+            if k < n_rev - 1:
                 ch, cw = sizes[k + 1]
                 b["gu"][k].add_(b["gub"][k])
                 H.call("vfs_adj_upsample", s, H.ptr(b["gu"][k + 1]), ch, cw, H.ptr(b["gu"][k]), hh, ww, 1,

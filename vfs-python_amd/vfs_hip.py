@@ -50,11 +50,11 @@ class VfsPdWs(ctypes.Structure):
     _fields_ = [(n, c_p) for n in ("u_th0", "u_th1", "u_bar2", "v_bar2", "p2", "q2")]
 
 
-class VfsAdjTiles(ctypes.Structure):
-    """struct vfs_adj_tiles (include/vfs_hip.h)."""
+class VfsAdjOpts(ctypes.Structure):
+    """struct vfs_adj_opts (include/vfs_hip.h)."""
 
     _fields_ = [(n, ctypes.c_int32) for n in ("fwd_tile_x", "fwd_tile_y", "fwd_k", "fwd_threads", "rec_tile_x",
-                                              "rec_tile_y", "rec_k", "rec_threads", "sweep")]
+                                              "rec_tile_y", "rec_k", "rec_threads", "sweep", "reverse_warps")]
 
 
 class VfsAdjPd(ctypes.Structure):
@@ -120,9 +120,9 @@ _SIGS = {
     "vfs_reduce_sum": [c_p, c_p, c_p, c_i, c_p, c_i64, c_i],
     "vfs_residual_norm": [c_p, c_p, c_p, c_i, c_p, c_p, c_p, c_i64, c_i],
     "vfs_adj_forward_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
-                              c_p, c_p, ctypes.POINTER(VfsAdjTiles)],
+                              c_p, c_p, ctypes.POINTER(VfsAdjOpts)],
     "vfs_adj_reverse_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
-                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, ctypes.POINTER(VfsAdjTiles)],
+                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, ctypes.POINTER(VfsAdjOpts)],
     "vfs_capture_begin": [ctypes.POINTER(c_p)],
     "vfs_capture_end": [c_p, ctypes.POINTER(c_p)],
     "vfs_plan_launch": [c_p, c_p],
