@@ -177,7 +177,8 @@ def run_fit(a):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "fit_512", "size": list(r["spec"]["size"]), "pairs": r["spec"]["pairs"],
-                   "chunk": r["spec"]["chunk"], "adam_steps": r["spec"]["steps"], "held_out": r["spec"]["held_out"],
+                   "pairs_per_step": r["spec"]["batch"], "adam_steps": r["spec"]["steps"],
+                   "held_out": r["spec"]["held_out"],
                    "schedule": fit.SCHEDULE},
         "fit_seconds": round(r["fit_seconds"], 3),
         "start": r["start"],

@@ -6,8 +6,10 @@ over the mask pixels whose ground-truth disparity is at least ``min_disparity``
 pixels (near the epipoles the disparity vanishes and triangulation is
 ill-conditioned: those pixels would dominate a mean of |log z - log z_gt|).  A
 step takes the mean loss and gradient over a chunk of pairs and updates the
-logarithms of the five parameters with Adam (they stay positive).  Pairs of a
-chunk are solved one after another on one stream and share one reverse workspace.
+logarithms of the five parameters with Adam (they stay positive).  Pairs are
+solved one after another on one stream and share one reverse workspace; at most
+``resident`` pairs keep their adjoint buffers on the GPU (least recently used
+first out), so a step over many pairs streams them through in chunks.
 
 The gradient is by default the truncated one (``reverse_warps`` = 1: the reverse
 of the last warp of the finest level, its start state held fixed).  The exact
@@ -17,13 +19,15 @@ the parameters is chaotic: the exact derivative is orders of magnitude larger
 than any finite-difference slope down to h = 1e-6 (relative), and it does not
 give descent at usable step sizes; the truncated one does.
 
-FIT_FULL is the full configuration (256 pairs at 1024 x 1024 in chunks of 32,
-200 Adam steps); FIT_DEMO is the scaled run that bench.py times.
+FIT_FULL is the full configuration (256 pairs at 1024 x 1024, every Adam step over
+all of them, 32 resident at a time, 200 steps); FIT_DEMO is the scaled run that
+bench.py times.
 
     python fit.py --spec demo          # on the GPU
 """
 
 import argparse
+import collections
 import json
 import math
 import sys
@@ -43,8 +47,9 @@ SCHEDULE = dict(nLevel=3, nWarpIters=10, nSolverIters=50, fScale=2.0, checkpoint
 # the reference's defaults (Python/vfs_params.py:33-43): the manual starting point
 START = dict(Lambda=5.0, alpha0=17.0, alpha1=1.2, beta=9.0, gamma=0.85)
 MIN_DISPARITY = 2.0
-FIT_FULL = dict(size=(1024, 1024), pairs=256, chunk=32, steps=200, held_out=32, lr=0.05, reverse_warps=1)
-FIT_DEMO = dict(size=(512, 512), pairs=16, chunk=16, steps=12, held_out=8, lr=0.05, reverse_warps=1)
+FIT_FULL = dict(size=(1024, 1024), pairs=256, batch=256, resident=32, steps=200, held_out=32, lr=0.05,
+                reverse_warps=1)
+FIT_DEMO = dict(size=(512, 512), pairs=16, batch=16, resident=16, steps=12, held_out=8, lr=0.05, reverse_warps=1)
 
 
 def scene_pairs(n, size, base_seed, device="cuda"):
@@ -128,17 +133,14 @@ class ParameterFit:
     """Adam on log(theta) with the mean depth-loss gradient over chunks of pairs."""
 
     def __init__(self, pairs, theta0=START, lr=0.05, betas=(0.9, 0.999), eps=1e-8, reverse_warps=1,
-                 min_disparity=MIN_DISPARITY):
+                 min_disparity=MIN_DISPARITY, resident=32):
         self.pairs = pairs
         self.ws = {}
-        self.adj = []
-        for pr in pairs:
-            self.adj.append(A.VFSDepthAdjoint(make_param(pr["img_ref"], theta0), pr["mask"], pr["tvf"], pr["cvf"],
-                                              pr["cam"], pr["t"], loss_target(pr, min_disparity),
-                                              checkpoint=SCHEDULE["checkpoint"], workspace=self.ws,
-                                              reverse_warps=reverse_warps))
-        self.refs = [torch.as_tensor(pr["img_ref"], device="cuda").to(torch.float32) / 256.0 for pr in pairs]
-        self.others = [torch.as_tensor(pr["img_other"], device="cuda") for pr in pairs]
+        self.theta0 = dict(theta0)
+        self.reverse_warps = int(reverse_warps)
+        self.min_disparity = float(min_disparity)
+        self.resident = max(1, int(resident))
+        self._adj = collections.OrderedDict()
         self.phi = np.log(np.array([theta0[k] for k in PARAMS], dtype=np.float64))
         self.m = np.zeros(5)
         self.v = np.zeros(5)
@@ -149,14 +151,31 @@ class ParameterFit:
     def theta(self):
         return {k: float(x) for k, x in zip(PARAMS, np.exp(self.phi))}
 
+    def adjoint(self, i):
+        """The adjoint of pair i with its device buffers (created on first use; the least
+        recently used one is released beyond `resident`)."""
+        if i in self._adj:
+            self._adj.move_to_end(i)
+            return self._adj[i]
+        pr = self.pairs[i]
+        a = A.VFSDepthAdjoint(make_param(pr["img_ref"], self.theta0), pr["mask"], pr["tvf"], pr["cvf"], pr["cam"],
+                              pr["t"], loss_target(pr, self.min_disparity), checkpoint=SCHEDULE["checkpoint"],
+                              workspace=self.ws, reverse_warps=self.reverse_warps)
+        ref = torch.as_tensor(pr["img_ref"], device="cuda").to(torch.float32) / 256.0
+        other = torch.as_tensor(pr["img_other"], device="cuda")
+        self._adj[i] = (a, ref, other)
+        while len(self._adj) > self.resident:
+            self._adj.popitem(last=False)
+        return self._adj[i]
+
     def loss_and_grad(self, idx):
         """Mean loss and d(mean loss)/d(log theta) over the pairs idx."""
         th = np.exp(self.phi)
         tot, g = 0.0, np.zeros(5)
         for i in idx:
-            a = self.adj[i]
+            a, ref, other = self.adjoint(i)
             a.set_params(th)
-            tot += a.forward(self.refs[i], self.others[i])
+            tot += a.forward(ref, other)
             gr = a.backward()
             g += np.array([gr[k] for k in PARAMS])
         n = len(idx)
@@ -173,11 +192,12 @@ class ParameterFit:
         self.history.append(dict(step=self.t, loss=loss, **self.theta()))
         return loss
 
-    def run(self, steps, chunk, log=None):
+    def run(self, steps, batch, log=None):
+        """`steps` Adam steps, each over `batch` pairs (consecutive, cycling through the pairs)."""
         n = len(self.pairs)
         for s in range(steps):
-            start = (s * chunk) % n
-            idx = [(start + j) % n for j in range(min(chunk, n))]
+            start = (s * batch) % n
+            idx = [(start + j) % n for j in range(min(batch, n))]
             loss = self.step(idx)
             if log:
                 log(f"[fit] step {self.t} loss {loss:.5f} " + " ".join(f"{k} {v:.4g}" for k, v in self.theta().items()))
@@ -191,16 +211,16 @@ def run_spec(spec, base_seed=3000, log=None):
     held = scene_pairs(spec["held_out"], spec["size"], base_seed + 100_000)
     t_gen = time.perf_counter() - t0
     err0, _, hl0 = median_depth_error(held, START)
-    f = ParameterFit(train, START, lr=spec["lr"], reverse_warps=spec["reverse_warps"])
+    f = ParameterFit(train, START, lr=spec["lr"], reverse_warps=spec["reverse_warps"], resident=spec["resident"])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    hist = f.run(spec["steps"], spec["chunk"], log=log)
+    hist = f.run(spec["steps"], spec["batch"], log=log)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t1
     best = min(f.history, key=lambda h: h["loss"])
     theta = f.theta()
     err1, _, hl1 = median_depth_error(held, theta)
-    grads = spec["steps"] * min(spec["chunk"], spec["pairs"])
+    grads = spec["steps"] * min(spec["batch"], spec["pairs"])
     return dict(spec=spec, start=START, fitted=theta, held_out_median_rel_depth_err_start=err0,
                 held_out_median_rel_depth_err_fitted=err1, held_out_loss_start=hl0, held_out_loss_fitted=hl1,
                 train_loss=[h["loss"] for h in hist], best_step=best["step"], fit_seconds=t_fit,
