@@ -14,6 +14,7 @@ Usage on the GPU:  python autotune.py --sizes 800x800,400x400,... --iters 50
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 
@@ -64,6 +65,18 @@ def tile_for(h, w, n_iter):
         best = min(t.items(), key=lambda kv: (abs(kv[0][0] * kv[0][1] - h * w), abs(kv[0][2] - n_iter)))
         return best[1]
     return default_tile(h, w)
+
+
+def tile_for_pixels(n, n_iter):
+    """The tuned variant of the table entry whose pixel count is nearest to n (log scale):
+    the choice for one launch over n pixels spread across many pairs."""
+    t = _load()
+    if not t:
+        side = int(math.sqrt(n))
+        return default_tile(side, side)
+    best = min(t.items(), key=lambda kv: (abs(math.log(kv[0][0] * kv[0][1]) - math.log(max(n, 1))),
+                                          abs(kv[0][2] - n_iter)))
+    return best[1]
 
 
 def time_variant(h, w, n_iter, var, reps=5, seed=0, graph=True, level=None):

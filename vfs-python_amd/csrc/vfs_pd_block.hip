@@ -506,6 +506,10 @@ const Variant kVariants[] = {
     VFS_PD_VARIANT(8, 8, 1, 128),    VFS_PD_VARIANT(8, 8, 2, 256),    VFS_PD_VARIANT(8, 8, 4, 512),
     VFS_PD_VARIANT(16, 8, 2, 256),   VFS_PD_VARIANT(16, 8, 4, 512),   VFS_PD_VARIANT(16, 16, 4, 512),
     VFS_PD_VARIANT(32, 8, 2, 256),   VFS_PD_VARIANT(32, 8, 4, 512),   VFS_PD_VARIANT(32, 16, 2, 512),
+    // tiles whose extended region fills the workgroup exactly (no idle lanes): 26 + 6 = 32,
+    // 29 + 3 = 32, 20 + 12 = 32, 58 + 6 = 64
+    VFS_PD_VARIANT(26, 26, 2, 1024), VFS_PD_VARIANT(29, 29, 1, 1024), VFS_PD_VARIANT(20, 20, 4, 1024),
+    VFS_PD_VARIANT(58, 26, 2, 1024), VFS_PD_VARIANT(26, 10, 2, 512),  VFS_PD_VARIANT(13, 13, 1, 256),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 

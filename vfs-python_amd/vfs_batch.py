@@ -52,15 +52,17 @@ class VFSBatch:
         self._plan_key = None
 
     def _tiles(self):
-        """One (tile_x, tile_y, k, threads) per pyramid level for all pairs: the tuned choice for the largest pair."""
+        """One (tile_x, tile_y, k, threads) per pyramid level for all pairs.  A batched launch
+        covers the tiles of every pair, so the choice is the tuned one for a level of the
+        same total pixel count."""
         n_level = len(self.vfs[0].sizes)
         out = []
         for lvl in range(n_level):
             if self.pd_tiles is not None and lvl in self.pd_tiles:
                 out.append(autotune.variant(self.pd_tiles[lvl]))
                 continue
-            h, w = max((v.sizes[lvl] for v in self.vfs), key=lambda s: s[0] * s[1])
-            out.append(autotune.tile_for(h, w, int(self.param.nSolverIters)))
+            total = sum(v.sizes[lvl][0] * v.sizes[lvl][1] for v in self.vfs)
+            out.append(autotune.tile_for_pixels(total, int(self.param.nSolverIters)))
         return out
 
     def solve(self, img_refs, img_others):
