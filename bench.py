@@ -426,7 +426,8 @@ def cpu_oracle_baseline(ctx, name):
     w = S.WORKLOADS[name]
     pair = ctx["pair"]
     Hh, Ww = pair["img_ref"].shape
-    oprm = O.OracleParams(Hh, Ww, nLevel=w["nLevel"], fScale=2.0, nWarpIters=1, nSolverIters=w["nSolverIters"],
+    oprm = O.OracleParams(Hh, Ww, nLevel=w["nLevel"], fScale=w.get("fScale", 2.0), nWarpIters=1,
+                          nSolverIters=w["nSolverIters"],
                           Lambda=w["Lambda"], alpha0=w["alpha0"], alpha1=w["alpha1"], beta=w["beta"],
                           gamma=w["gamma"])
     t0 = time.perf_counter()
@@ -459,7 +460,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain_full_800",
-                    choices=["plain_full_800", "plain_small_256", "batched_64", "adjoint_800", "fit_512"])
+                    choices=["plain_full_800", "plain_small_256", "default_800", "batched_64", "adjoint_800",
+                             "fit_512"])
     ap.add_argument("--pairs", type=int, default=64, help="batched_64: number of pairs (first n of the 64)")
     ap.add_argument("--solver", default="blocked", choices=["graph", "eager", "blocked", "blocked_eager"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain per-kernel path beside it")
@@ -517,14 +519,16 @@ def main():
 
     dt, xyz = timed(a.solver)
     maps_s = a.steps / dt
+    launch_mode = v.launch_mode
     flow_main = v.warpUV.cpu().numpy()
     plain = None
     if not a.no_compare and a.solver not in ("graph", "eager"):
         v.solver = "graph"
-        dtp, _ = timed("plain per-kernel (graph)")
+        dtp, _ = timed("plain per-kernel")
         same = bool(np.array_equal(v.warpUV.cpu().numpy(), flow_main))
         plain = dict(value=round(a.steps / dtp, 3), ms_per_step=round(1e3 * dtp / a.steps, 3),
-                     speedup_of_reported=round(dtp / dt, 2), flow_bitwise_equal=same)
+                     speedup_of_reported=round(dtp / dt, 2), flow_bitwise_equal=same,
+                     launches=v.launch_count(), launch_mode=v.launch_mode)
         v.solver = a.solver
 
     # accuracy of the timed result against the synthetic ground truth (not a parity check)
@@ -573,8 +577,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": a.workload, "size": [w["H"], w["W"]], "levels": w["nLevel"],
+                   "level_ratio": w.get("fScale", 2.0),
                    "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
-                   "solver": a.solver, "pairs": 1,
+                   "solver": a.solver, "launch_mode": launch_mode, "launches_per_map": v.launch_count(),
+                   "pairs": 1,
                    "pd_tiles": [list(t) for t in v._tiles()] if a.solver.startswith("blocked") else None},
         "plain_per_kernel": plain,
         "mpx_iter_per_s": round(px_iter * maps_s / 1e6, 1),

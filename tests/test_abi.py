@@ -121,3 +121,27 @@ def test_synthetic_generator_is_seeded_and_masked():
     pyr = O.create_mask_pyramid(m, O.compute_pyramid_sizes(64, 64, 3, 2.0))
     for lvl in pyr:  # zero outer ring at every level
         assert lvl[0].max() == 0 and lvl[-1].max() == 0 and lvl[:, 0].max() == 0 and lvl[:, -1].max() == 0
+
+
+def test_launch_count_and_graph_bound():
+    """Launches per solve: the per-kernel solve issues 6 + 3 n_iter per warp, the
+    blocked one 2 + ceil(n_iter / k); solves above the graph bound launch eagerly."""
+    import vfs as V
+    import vfs_params as P
+    import synthetic as S
+    prm = P.vfs_params(np.zeros((800, 800), np.uint8))
+    v = V.VFS(prm)
+    v.compute_pyramid_sizes()
+    v.solver = "graph"
+    n_plain = v.launch_count()
+    assert n_plain == 4 * 11 + 4 + 11 * (3 + 100 * (6 + 3 * 100))
+    assert n_plain > V.GRAPH_MAX_LAUNCHES
+    v.solver = "blocked"
+    v.pd_tiles = {lvl: (16, 16, 2) for lvl in range(11)}
+    assert v.launch_count() == 4 * 11 + 4 + 11 * (3 + 100 * (2 + 50))
+    v.pd_tiles = None
+    assert v.launch_count() <= V.GRAPH_MAX_LAUNCHES  # the default parameters' blocked solve is recorded
+    S.apply_workload_params(prm, "plain_full_800")
+    v.compute_pyramid_sizes()
+    v.solver = "graph"
+    assert v.launch_count() <= V.GRAPH_MAX_LAUNCHES

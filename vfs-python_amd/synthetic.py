@@ -213,17 +213,18 @@ def circle_mask(H, W, margin, radius=None):
 
 def make_pair(H, W, seed, t=(0.2, 0.0, 0.0), rot_deg=1.0, rot_random=False, n_levels=3,
               depth_base=4.0, depth_clip=(1.0, 10.0), contrast=1.0, noise_std=1.0 / 255.0,
-              fp_iters=30, device=None):
+              fp_iters=30, device=None, level_ratio=2.0, f=None):
     """Return a dict describing one seeded synthetic pair.
 
     Keys: img_ref, img_other (H, W) uint8; mask (H, W) uint8 in {0, 255}; range_gt, z_gt
-    (H, W) float64; R (3, 3); t (3, 1); cam (f, cx, cy).
+    (H, W) float64; R (3, 3); t (3, 1); cam (f, cx, cy).  The focal length defaults to
+    min(H, W) / pi (a 180 deg image circle).
     ``rot_random`` draws the rotation angle uniformly in [0, rot_deg].  ``device``
     (e.g. "cuda") evaluates the per-pixel math with torch on that device.
     """
     xp = _NumpyOps if device is None else _TorchOps(device)
     rng = np.random.Generator(np.random.PCG64(seed))
-    f = min(H, W) / math.pi
+    f = min(H, W) / math.pi if f is None else float(f)
     cx, cy = W / 2.0, H / 2.0
     axis = rng.normal(size=3)
     ang = math.radians(rng.uniform(0, rot_deg) if rot_random else rot_deg)
@@ -231,7 +232,7 @@ def make_pair(H, W, seed, t=(0.2, 0.0, 0.0), rot_deg=1.0, rot_random=False, n_le
     tv = np.asarray(t, dtype=np.float64).reshape(3)
     depth = RangeField(rng, H, W, base=depth_base, clip=depth_clip)
     tex = ValueNoise(rng, base_cell=max(8.0, min(H, W) / 8.0))
-    margin = 2 ** (n_levels - 1) + 1
+    margin = int(math.ceil(level_ratio ** (n_levels - 1))) + 1  # one coarsest-level pixel
 
     X, Y = xp.grid(H, W)
     lo = 0.5 - 0.5 * contrast
@@ -282,6 +283,11 @@ WORKLOADS = {
     # gradient of the depth loss: 3 levels x 10 warps x 50 iterations, checkpoint every 25
     "adjoint_800": dict(H=800, W=800, t=(0.3, 0.0, 0.0), rot_deg=2.0, rot_random=True,
                         nLevel=3, nWarpIters=10, nSolverIters=50, checkpoint=25, pairs=8, **_SOLVER),
+    # the reference's default parameters (Python/vfs_params.py:25-43): 11 levels at a
+    # ratio of 1.2 (800x800 down to 127x127), 100 warps x 100 iterations per level
+    "default_800": dict(H=800, W=800, t=(0.3, 0.0, 0.0), rot_deg=2.0, rot_random=True,
+                        nLevel=11, fScale=1.2, nWarpIters=100, nSolverIters=100,
+                        Lambda=5.0, alpha0=17.0, alpha1=1.2, beta=9.0, gamma=0.85),
 }
 
 # 64 pairs, 16 each of four sizes, every pair with its own pose and scene.
@@ -307,7 +313,7 @@ def batch_pairs(spec=BATCH_64, base_seed=1000, device=None, n=None):
 def workload_pair(name, seed=0, device=None):
     w = WORKLOADS[name]
     return make_pair(w["H"], w["W"], seed, t=w["t"], rot_deg=w["rot_deg"], rot_random=w["rot_random"],
-                     n_levels=w["nLevel"], device=device)
+                     n_levels=w["nLevel"], device=device, level_ratio=w.get("fScale", 2.0))
 
 
 def sanitize_field(tvf):
@@ -325,9 +331,10 @@ def sanitize_field(tvf):
 
 
 def apply_workload_params(param, name_or_spec):
-    """Set the solver fields of a vfs_params-like object for a named workload (fScale 2)."""
+    """Set the solver fields of a vfs_params-like object for a named workload (fScale 2
+    unless the workload names another)."""
     w = WORKLOADS[name_or_spec] if isinstance(name_or_spec, str) else name_or_spec
     for k in ("nLevel", "nWarpIters", "nSolverIters", "Lambda", "alpha0", "alpha1", "beta", "gamma"):
         setattr(param, k, w[k])
-    param.fScale = 2.0
+    param.fScale = w.get("fScale", 2.0)
     return param

@@ -24,6 +24,11 @@ attribute ``solver`` selects how it is driven:
   enqueued per call.  The tile shape and k per pyramid level come from
   ``pd_tiles`` (a dict {level: (tile_x, tile_y, k[, threads])}) or from the autotuned
   table (autotune.py).
+
+A recorded solve holds at most ``GRAPH_MAX_LAUNCHES`` kernel launches; a
+larger one (the per-kernel solve at the default 11 levels x 100 warps x 100
+iterations issues ~337 k) is enqueued eagerly instead, with the same kernels
+and bitwise the same result.  ``launch_mode`` tells which ran.
 """
 
 import ctypes
@@ -36,6 +41,13 @@ import texture
 import vfs_hip as H
 
 _DEV = "cuda"
+
+# Largest solve recorded into one HIP graph.  Recorded solves of up to ~47 k
+# launches (the blocked solve at the default parameters, 800x800) replay normally; the
+# recording of a ~337 k-launch per-kernel solve ended the process with a
+# segmentation fault inside graph capture / instantiation, so solves above this
+# bound are launched eagerly.
+GRAPH_MAX_LAUNCHES = 50_000
 
 
 def _u8_device(img):
@@ -67,6 +79,7 @@ class VFS:
         self.residuals = []
         self.pd_tiles = None
         self.median_filter_kernel_size = 5
+        self.launch_mode = None  # "graph" or "eager": how the last solve_stereo ran
         self._plan = None
         self._plan_key = None
         self._bufs = None
@@ -300,15 +313,18 @@ class VFS:
         """Coarse-to-fine solve; returns the flow (H, W, 2) and stores it as ``warpUV``."""
         self.setup_module()
         if self.solver == "python":
+            self.launch_mode = "python"
             return self._solve_stereo_python(img_ref, img_other)
         if self.solver not in ("graph", "eager", "blocked", "blocked_eager"):
             raise ValueError(f"unknown solver {self.solver!r}")
         b = self._buffers()
         b["img_ref"].copy_(_u8_device(img_ref))
         b["img_other"].copy_(_u8_device(img_other))
-        if self.solver in ("eager", "blocked_eager"):
+        if self.solver in ("eager", "blocked_eager") or self.launch_count() > GRAPH_MAX_LAUNCHES:
+            self.launch_mode = "eager"
             self._enqueue_solve(H.stream(), b)
         else:
+            self.launch_mode = "graph"
             key = self._key(b)
             if self._plan is None or self._plan_key != key:
                 self.release_plan()
@@ -346,6 +362,18 @@ class VFS:
                 float(p.limitRange), float(p.fScale), self.median_filter_kernel_size,
                 tuple(m.data_ptr() for m in self.masks), tuple(t.data_ptr() for t in self.translations),
                 self.calibration.data_ptr(), id(b), self.solver, tuple(self._tiles()))
+
+    def launch_count(self):
+        """Kernel launches of one solve_stereo with the current sizes, solver and schedule."""
+        p = self.param
+        n_warp, n_iter = int(p.nWarpIters), int(p.nSolverIters)
+        n = 4 * len(self.sizes) + 4  # pyramids, level starts, result
+        blocked = self.solver in ("blocked", "blocked_eager")
+        tiles = self._tiles() if blocked else [None] * len(self.sizes)
+        for t in tiles:
+            n += 3
+            n += n_warp * ((2 + -(-n_iter // t[2])) if blocked else (6 + 3 * n_iter))
+        return n
 
     def _tiles(self):
         """(tile_x, tile_y, k, threads) of the blocked primal-dual loop for every pyramid level."""
