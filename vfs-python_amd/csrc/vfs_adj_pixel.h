@@ -7,8 +7,9 @@
 // threshold^T, dual^T.  A contribution to a neighbour's adjoint is written as a
 // per-pixel message and collected by the receiving pixel, never scattered.
 //
-// Non-smooth points: projections are active when the norm exceeds 1; the
-// threshold uses the branch its forward took.
+// Non-smooth points: projections are active when the squared norm exceeds 1; the
+// threshold uses the branch its forward took.  Forward values are recomputed with
+// the same arithmetic as vfs_pd.h (reciprocal steps, rsqrt projection scale).
 #pragma once
 
 #include "vfs_device.h"
@@ -125,8 +126,8 @@ __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4
     }
     const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
     const bool d1 = !(fabsf(e1) < kEps), d2 = !(fabsf(e2) < kEps);
-    const float t1 = d1 ? tau / e1 : tau;
-    const float t2 = d2 ? tau / e2 : tau;
+    const float t1 = tau * recip_eta(e1, kEps);
+    const float t2 = tau * recip_eta(e2, kEps);
     const float S1 = gQ.x + gQ.z, S2 = gQ.w + gQ.y;
     const float A1 = fmaf(alpha1, tu.x, alpha0 * S1), A2 = fmaf(alpha1, tu.y, alpha0 * S2);
     // ub_out = (1 + mu) u - mu ub ; vb_out = (1 + mu) v - mu vb ; v = t A + vb
@@ -207,7 +208,7 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
     const float gy = (n.mS && n.mN) ? tc.y - tpNy : (!n.mS ? -tpNy : tc.y);
     const float dIu = d.x, dIz = d.y;
     const bool de = !(fabsf(e0) < kEps);
-    const float te = de ? tau / e0 : tau;
+    const float te = tau * recip_eta(e0, kEps);
     const float ut = ubc + te * (gx + gy);
     const float rho = dIu * (ut - uc) + dIz;
     const float bound = lambda * te * dIu * dIu;
@@ -216,10 +217,11 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
     if ((rho <= bound) && (rho >= -bound)) {
         if (fabsf(dIu) < kEps) {
             g_d0 = gout;
-        } else {  // du = d0 - rho / dIu
-            const float g_rho = -gout / dIu;
+        } else {  // du = d0 - rho r, r = 1 / dIu
+            const float r = recip_grad(dIu, kEps);
+            const float g_rho = -gout * r;
             g_d0 = gout + g_rho * dIu;
-            g_dIu = gout * rho / (dIu * dIu) + g_rho * d0;
+            g_dIu = gout * rho * r * r + g_rho * d0;
             g_dIz = g_rho;
         }
     } else if (rho < -bound) {  // du = d0 + lambda te dIu
@@ -352,14 +354,17 @@ __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float
     const float sp = dual_scale(alpha1, sigma, eta_p), sq = dual_scale(alpha0, sigma, eta_q);
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
     const float2 up = make_float2(fmaf(sp, tu.x, pc.x), fmaf(sp, tu.y, pc.y));
-    const float np_ = hypotf(up.x, up.y);
-    const bool projp = fmaxf(np_, 1.0f) > 1.0f;
-    const float2 pn = projp ? make_float2(up.x / np_, up.y / np_) : up;
+    // the forward projections (dual_pixel): scale r = 1 / |x| where |x|^2 > 1
+    const float s2 = sq_norm2(up.x, up.y);
+    const bool projp = s2 > 1.0f;
+    const float rp = unit_ball_scale(s2);
+    const float2 pn = make_float2(up.x * rp, up.y * rp);
     const float4 uq = make_float4(fmaf(sq, gV.x, qc.x), fmaf(sq, gV.y, qc.y), fmaf(sq, gV.z, qc.z),
                                   fmaf(sq, gV.w, qc.w));
-    const float nq = norm4df(uq.x, uq.y, uq.z, uq.w);
-    const bool projq = fmaxf(nq, 1.0f) > 1.0f;
-    const float4 qn = projq ? make_float4(uq.x / nq, uq.y / nq, uq.z / nq, uq.w / nq) : uq;
+    const float s4 = sq_norm4(uq.x, uq.y, uq.z, uq.w);
+    const bool projq = s4 > 1.0f;
+    const float rq = unit_ball_scale(s4);
+    const float4 qn = make_float4(uq.x * rq, uq.y * rq, uq.z * rq, uq.w * rq);
     // tp = T pn
     float ga = gtp.x * pn.x, gb = gtp.y * pn.y, gc = gtp.x * pn.y + gtp.y * pn.x;
     gpn.x += a * gtp.x + c * gtp.y;
@@ -368,13 +373,13 @@ __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float
     float2 gup = gpn;
     if (projp) {
         const float dp = pn.x * gpn.x + pn.y * gpn.y;
-        gup = make_float2((gpn.x - pn.x * dp) / np_, (gpn.y - pn.y * dp) / np_);
+        gup = make_float2((gpn.x - pn.x * dp) * rp, (gpn.y - pn.y * dp) * rp);
     }
     float4 guq = gqn;
     if (projq) {
         const float dq = qn.x * gqn.x + qn.y * gqn.y + qn.z * gqn.z + qn.w * gqn.w;
-        guq = make_float4((gqn.x - qn.x * dq) / nq, (gqn.y - qn.y * dq) / nq, (gqn.z - qn.z * dq) / nq,
-                          (gqn.w - qn.w * dq) / nq);
+        guq = make_float4((gqn.x - qn.x * dq) * rq, (gqn.y - qn.y * dq) * rq, (gqn.z - qn.z * dq) * rq,
+                          (gqn.w - qn.w * dq) * rq);
     }
     gp_out = gup;  // d up / d pc = I
     gq_out = guq;

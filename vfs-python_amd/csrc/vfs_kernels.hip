@@ -224,8 +224,8 @@ __global__ void __launch_bounds__(256) k_l1_thresholding(float* out, float tau, 
     const bool mE = mask[Y * W + clamp_idx(X + 1, W)] != 0, mW = mask[iW] != 0;
     const bool mS = mask[clamp_idx(Y + 1, H) * W + X] != 0, mN = mask[iN] != 0;
     const float2 d = deriv[idx];
-    out[idx] = l1_pixel(tp[idx], tp[iW].x, tp[iN].y, mE, mW, mS, mN, d.x, d.y, eta[3 * idx], u[idx], u_bar[idx], tau,
-                        lambda, kEps);
+    out[idx] = l1_pixel(tp[idx], tp[iW].x, tp[iN].y, mE, mW, mS, mN, d.x, recip_grad(d.x, kEps), d.y,
+                        recip_eta(eta[3 * idx], kEps), u[idx], u_bar[idx], tau, lambda);
 }
 
 // updatePrimalVariables, Python/vfs.cu:507-581.
@@ -250,8 +250,8 @@ __global__ void __launch_bounds__(256) k_update_primal(float* u_out, float2* v_o
     float ub;
     float2 vb;
     primal_pixel(q[idx], qW, qN, mE, mW, mS, mN, abc[3 * idx + 0], abc[3 * idx + 1], abc[3 * idx + 2], p[idx],
-                 eta[3 * idx + 1], eta[3 * idx + 2], v_bar[idx], u[idx], u_bar[idx], tau, mu, alpha0, alpha1, kEps, ub,
-                 vb);
+                 recip_eta(eta[3 * idx + 1], kEps), recip_eta(eta[3 * idx + 2], kEps), v_bar[idx], u[idx], u_bar[idx],
+                 tau, mu, alpha0, alpha1, ub, vb);
     u_out[idx] = ub;
     v_out[idx] = vb;
 }

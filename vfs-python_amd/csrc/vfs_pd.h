@@ -1,14 +1,54 @@
 // Per-pixel arithmetic of one TGV-L1 primal-dual iteration, shared by the
-// per-kernel path (vfs_kernels.hip) and the temporally blocked path
-// (vfs_pd_block.hip) so that both evaluate the same operation sequence.
+// per-kernel path (vfs_kernels.hip), the temporally blocked path
+// (vfs_pd_block.hip) and the adjoint's recomputation (vfs_adj_pixel.h), so that
+// all evaluate the same operation sequence.
 //
 // Dual step: Python/vfs.cu:326-409.  Threshold: Python/vfs.cu:421-495.
 // Primal step: Python/vfs.cu:507-570.
+//
+// Arithmetic form.  The reference divides by the per-pixel metric eta and by the
+// image derivative Iu at every iteration, and projects with hypotf / norm4df and a
+// division per component.  Here the divisions by per-pixel constants are products
+// with their reciprocals (recip_eta, recip_grad), formed once where the constants
+// are loaded, and the projection multiplies by rsqrt of the squared norm
+// (unit_ball_scale).  Each differs from the quotient by at most a couple of ulp;
+// the iteration is held to the float64 oracle with the same tolerances.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace vfs {
+
+// 1 / eta for the metric steps tau / eta (Python/vfs.cu:430-431,518-521): the step
+// is tau * recip_eta(eta), and tau itself where |eta| < eps.
+__host__ __device__ __forceinline__ float recip_eta(float e, float eps)
+{
+    return (fabsf(e) < eps) ? 1.0f : 1.0f / e;
+}
+
+// 1 / Iu for the threshold's inner branch (Python/vfs.cu:476-480), 0 where
+// |Iu| < eps so that the branch reduces to the unthresholded step.
+__host__ __device__ __forceinline__ float recip_grad(float d, float eps)
+{
+    return (fabsf(d) < eps) ? 0.0f : 1.0f / d;
+}
+
+// Scale of the projection onto the unit ball given the squared norm s:
+// 1 / sqrt(s) where s > 1, else 1 (x * 1 is x exactly).
+__device__ __forceinline__ float unit_ball_scale(float s)
+{
+    return s > 1.0f ? __builtin_amdgcn_rsqf(s) : 1.0f;
+}
+
+__device__ __forceinline__ float sq_norm2(float x, float y)
+{
+    return fmaf(x, x, y * y);
+}
+
+__device__ __forceinline__ float sq_norm4(float x, float y, float z, float w)
+{
+    return fmaf(x, x, fmaf(y, y, fmaf(z, z, w * w)));
+}
 
 // The two step sizes of the dual ascent, alpha1 sigma / eta_p and alpha0 sigma /
 // eta_q (Python/vfs.cu:387,397), as the reference forms them in fp32.
@@ -21,8 +61,7 @@ __host__ __device__ __forceinline__ float dual_scale(float alpha, float sigma, f
 // for each axis: (uX, vX) is the east neighbour when fwd_x (forward difference)
 // and the west neighbour otherwise; likewise (uY, vY) south / north.
 // scale_p / scale_q are dual_scale(alpha1, sigma, eta_p) / dual_scale(alpha0, sigma, eta_q).
-// The projection divides by max(norm, 1); division by 1 is exact, so it is
-// skipped when the norm does not exceed 1.
+// The projection scales by 1 / max(norm, 1) (unit_ball_scale).
 __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2 vX, bool fwd_x, float uY, float2 vY,
                                            bool fwd_y, float a, float b, float c, float2 pc, float4 qc, float scale_p,
                                            float scale_q, float2& pn, float2& tpn, float4& qn)
@@ -49,30 +88,21 @@ __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2
     }
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
     const float2 up = make_float2(fmaf(scale_p, tu.x, pc.x), fmaf(scale_p, tu.y, pc.y));
-    float nrm = fmaxf(hypotf(up.x, up.y), 1.0f);
-    pn = up;
-    if (nrm > 1.0f) {
-        pn.x = up.x / nrm;
-        pn.y = up.y / nrm;
-    }
+    const float rp = unit_ball_scale(sq_norm2(up.x, up.y));
+    pn = make_float2(up.x * rp, up.y * rp);
     tpn = make_float2(fmaf(a, pn.x, c * pn.y), fmaf(c, pn.x, b * pn.y));
     const float4 uq = make_float4(fmaf(scale_q, gV.x, qc.x), fmaf(scale_q, gV.y, qc.y), fmaf(scale_q, gV.z, qc.z),
                                   fmaf(scale_q, gV.w, qc.w));
-    nrm = fmaxf(norm4df(uq.x, uq.y, uq.z, uq.w), 1.0f);
-    qn = uq;
-    if (nrm > 1.0f) {
-        qn.x = uq.x / nrm;
-        qn.y = uq.y / nrm;
-        qn.z = uq.z / nrm;
-        qn.w = uq.w / nrm;
-    }
+    const float rq = unit_ball_scale(sq_norm4(uq.x, uq.y, uq.z, uq.w));
+    qn = make_float4(uq.x * rq, uq.y * rq, uq.z * rq, uq.w * rq);
 }
 
 // Linearised-data-term threshold: the prox of lambda |Iu (x - u) + Iz| in the
 // metric eta.x / tau, after the divergence step of T p.  Returns the new u.
+// r0 = recip_eta(eta.x), rIu = recip_grad(Iu).
 __device__ __forceinline__ float l1_pixel(float2 tc, float tpWx, float tpNy, bool mE, bool mW, bool mS, bool mN,
-                                          float dIu, float dIz, float e0, float uc, float ubc, float tau,
-                                          float lambda, float eps)
+                                          float dIu, float rIu, float dIz, float r0, float uc, float ubc, float tau,
+                                          float lambda)
 {
     float gx, gy;
     if (mE && mW)
@@ -87,13 +117,13 @@ __device__ __forceinline__ float l1_pixel(float2 tc, float tpWx, float tpNy, boo
         gy = -tpNy;
     else
         gy = tc.y;
-    const float tauEtaU = (fabsf(e0) < eps) ? tau : tau / e0;
+    const float tauEtaU = tau * r0;
     const float uTauGtp = ubc + tauEtaU * (gx + gy);
     const float rho = dIu * (uTauGtp - uc) + dIz;
     const float bound = lambda * tauEtaU * dIu * dIu;
     float du = 0.0f;
     if ((rho <= bound) && (rho >= -bound))
-        du = (fabsf(dIu) < eps) ? (uTauGtp - uc) : ((uTauGtp - uc) - rho / dIu);
+        du = (uTauGtp - uc) - rho * rIu;
     else if (rho < -bound)
         du = (uTauGtp - uc) + lambda * tauEtaU * dIu;
     else if (rho > bound)
@@ -102,11 +132,12 @@ __device__ __forceinline__ float l1_pixel(float2 tc, float tpWx, float tpNy, boo
 }
 
 // Primal descent on v and over-relaxation of u and v.  qW / qN are read only
-// when the corresponding neighbour is inside the mask.
+// when the corresponding neighbour is inside the mask.  r1, r2 = recip_eta(eta.y),
+// recip_eta(eta.z).
 __device__ __forceinline__ void primal_pixel(float4 qc, float4 qW, float4 qN, bool mE, bool mW, bool mS, bool mN,
-                                             float a, float b, float c, float2 pc, float e1, float e2, float2 vb,
+                                             float a, float b, float c, float2 pc, float r1, float r2, float2 vb,
                                              float uc, float ub, float tau, float mu, float alpha0, float alpha1,
-                                             float eps, float& ub_out, float2& vb_out)
+                                             float& ub_out, float2& vb_out)
 {
     float4 gQ = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (mE) {
@@ -126,8 +157,8 @@ __device__ __forceinline__ void primal_pixel(float4 qc, float4 qW, float4 qN, bo
         gQ.z -= qN.z;
     }
     const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
-    const float tauEta1 = (fabsf(e1) < eps) ? tau : (tau / e1);
-    const float tauEta2 = (fabsf(e2) < eps) ? tau : (tau / e2);
+    const float tauEta1 = tau * r1;
+    const float tauEta2 = tau * r2;
     const float vx = fmaf(tauEta1, fmaf(alpha1, tu.x, alpha0 * (gQ.x + gQ.z)), vb.x);
     const float vy = fmaf(tauEta2, fmaf(alpha1, tu.y, alpha0 * (gQ.w + gQ.y)), vb.y);
     ub_out = fmaf(mu, uc - ub, uc);

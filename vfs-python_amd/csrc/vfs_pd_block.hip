@@ -117,7 +117,8 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
     const int ox = tx * TX - 2 * K, oy = ty * TY - 2 * K;
 
     float u[S], ub[S], vbx[S], vby[S], px[S], py[S], qx[S], qy[S], qz[S], qw[S];
-    float ca[S], cb[S], cc[S], e0[S], e1[S], e2[S], iu[S], iz[S];
+    // constants: abc, 1/eta (recip_eta), Iu, 1/Iu (recip_grad), Iz
+    float ca[S], cb[S], cc[S], r0[S], r1[S], r2[S], iu[S], riu[S], iz[S];
     unsigned fl[S];
 
 #pragma unroll
@@ -127,7 +128,7 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
         const int gx = ox + ex, gy = oy + ey;
         unsigned f = 0;
         u[j] = ub[j] = vbx[j] = vby[j] = px[j] = py[j] = qx[j] = qy[j] = qz[j] = qw[j] = 0.0f;
-        ca[j] = cb[j] = cc[j] = e0[j] = e1[j] = e2[j] = iu[j] = iz[j] = 0.0f;
+        ca[j] = cb[j] = cc[j] = r0[j] = r1[j] = r2[j] = iu[j] = riu[j] = iz[j] = 0.0f;
         if (e < E && gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const int idx = gy * W + gx;
             f |= F_IMG;
@@ -164,11 +165,12 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
                 ca[j] = L.abc[3 * idx + 0];
                 cb[j] = L.abc[3 * idx + 1];
                 cc[j] = L.abc[3 * idx + 2];
-                e0[j] = L.eta[3 * idx + 0];
-                e1[j] = L.eta[3 * idx + 1];
-                e2[j] = L.eta[3 * idx + 2];
+                r0[j] = recip_eta(L.eta[3 * idx + 0], kEps);
+                r1[j] = recip_eta(L.eta[3 * idx + 1], kEps);
+                r2[j] = recip_eta(L.eta[3 * idx + 2], kEps);
                 const float2 d = reinterpret_cast<const float2*>(L.deriv)[idx];
                 iu[j] = d.x;
+                riu[j] = recip_grad(d.x, kEps);
                 iz[j] = d.y;
             }
         }
@@ -240,16 +242,16 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
             if (f & F_ACT) {
                 const int eW = (f & F_HW) ? e - 1 : e, eN = (f & F_HN) ? e - EX : e;
                 const bool mE = (f & F_ME) != 0, mW = (f & F_MW) != 0, mS = (f & F_MS) != 0, mN = (f & F_MN) != 0;
-                u[j] = l1_pixel(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j], iz[j],
-                                e0[j], u[j], ub[j], tau, cst.lambda, kEps);
+                u[j] = l1_pixel(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j], riu[j],
+                                iz[j], r0[j], u[j], ub[j], tau, cst.lambda);
                 const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 const float4 qW = mW ? make_float4(sB[2 * E + eW], sB[3 * E + eW], sB[4 * E + eW], sB[5 * E + eW]) : zero;
                 const float4 qN = mN ? make_float4(sB[2 * E + eN], sB[3 * E + eN], sB[4 * E + eN], sB[5 * E + eN]) : zero;
                 float ubn;
                 float2 vbn;
                 primal_pixel(make_float4(qx[j], qy[j], qz[j], qw[j]), qW, qN, mE, mW, mS, mN, ca[j], cb[j], cc[j],
-                             make_float2(px[j], py[j]), e1[j], e2[j], make_float2(vbx[j], vby[j]), u[j], ub[j], tau,
-                             mu, cst.alpha0, cst.alpha1, kEps, ubn, vbn);
+                             make_float2(px[j], py[j]), r1[j], r2[j], make_float2(vbx[j], vby[j]), u[j], ub[j], tau,
+                             mu, cst.alpha0, cst.alpha1, ubn, vbn);
                 ub[j] = ubn;
                 vbx[j] = vbn.x;
                 vby[j] = vbn.y;
