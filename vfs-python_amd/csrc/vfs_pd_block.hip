@@ -512,6 +512,10 @@ const Variant kVariants[] = {
     // 29 + 3 = 32, 20 + 12 = 32, 58 + 6 = 64
     VFS_PD_VARIANT(26, 26, 2, 1024), VFS_PD_VARIANT(29, 29, 1, 1024), VFS_PD_VARIANT(20, 20, 4, 1024),
     VFS_PD_VARIANT(58, 26, 2, 1024), VFS_PD_VARIANT(26, 10, 2, 512),  VFS_PD_VARIANT(13, 13, 1, 256),
+    // exact fits for three iterations (23 + 9 = 32, 55 + 9 = 64) and a region of two pixels
+    // per lane for four (52 + 12 = 64 by 20 + 12 = 32); four pixels per lane (52 x 52) spills
+    VFS_PD_VARIANT(23, 23, 3, 1024), VFS_PD_VARIANT(55, 23, 3, 1024), VFS_PD_VARIANT(23, 7, 3, 512),
+    VFS_PD_VARIANT(52, 20, 4, 1024),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
