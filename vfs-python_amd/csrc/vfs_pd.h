@@ -9,9 +9,9 @@
 // Arithmetic form.  The reference divides by the per-pixel metric eta and by the
 // image derivative Iu at every iteration, and projects with hypotf / norm4df and a
 // division per component.  Here the divisions by per-pixel constants are products
-// with their reciprocals (recip_eta, recip_grad), formed once where the constants
-// are loaded, and the projection multiplies by rsqrt of the squared norm
-// (unit_ball_scale).  Each differs from the quotient by at most a couple of ulp;
+// with their reciprocals (recip_eta, recip_grad; the hardware reciprocal), formed
+// once where the constants are loaded, and the projection multiplies by rsqrt of the
+// squared norm (unit_ball_scale).  Each differs from the quotient by at most a couple of ulp;
 // the iteration is held to the float64 oracle with the same tolerances.
 #pragma once
 
@@ -20,17 +20,18 @@
 namespace vfs {
 
 // 1 / eta for the metric steps tau / eta (Python/vfs.cu:430-431,518-521): the step
-// is tau * recip_eta(eta), and tau itself where |eta| < eps.
-__host__ __device__ __forceinline__ float recip_eta(float e, float eps)
+// is tau * recip_eta(eta), and tau itself where |eta| < eps.  v_rcp_f32 (1 ulp):
+// eta >= eps keeps both the input and the result normal.
+__device__ __forceinline__ float recip_eta(float e, float eps)
 {
-    return (fabsf(e) < eps) ? 1.0f : 1.0f / e;
+    return (fabsf(e) < eps) ? 1.0f : __builtin_amdgcn_rcpf(e);
 }
 
 // 1 / Iu for the threshold's inner branch (Python/vfs.cu:476-480), 0 where
 // |Iu| < eps so that the branch reduces to the unthresholded step.
-__host__ __device__ __forceinline__ float recip_grad(float d, float eps)
+__device__ __forceinline__ float recip_grad(float d, float eps)
 {
-    return (fabsf(d) < eps) ? 0.0f : 1.0f / d;
+    return (fabsf(d) < eps) ? 0.0f : __builtin_amdgcn_rcpf(d);
 }
 
 // Scale of the projection onto the unit ball given the squared norm s:

@@ -147,10 +147,6 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
             }
             if (L.mask[idx] != 0) {
                 f |= F_ACT;
-                if (L.mask[gy * W + clamp_idx(gx + 1, W)] != 0) f |= F_ME;
-                if (L.mask[gy * W + clamp_idx(gx - 1, W)] != 0) f |= F_MW;
-                if (L.mask[clamp_idx(gy + 1, H) * W + gx] != 0) f |= F_MS;
-                if (L.mask[clamp_idx(gy - 1, H) * W + gx] != 0) f |= F_MN;
                 u[j] = L.u_in[idx];
                 if (!L.first) {
                     const float2 p = reinterpret_cast<const float2*>(L.p_in)[idx];
@@ -179,6 +175,7 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
             sA[e] = ub[j];
             sA[E + e] = vbx[j];
             sA[2 * E + e] = vby[j];
+            sB[e] = (f & F_ACT) ? 1.0f : 0.0f;  // the mask, staged for the neighbour flags
         }
     }
 
@@ -188,6 +185,22 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
 #pragma unroll
     for (int j = 0; j < S; ++j) any_active |= static_cast<int>(fl[j] & F_ACT);
     const bool skip = __syncthreads_or(any_active) == 0 && k_count > 0;
+    // Neighbour mask flags from the staged mask.  A neighbour outside the loaded region
+    // is the pixel itself: the mirror at the image border, and at the region's own
+    // border a pixel whose state is inexact after the first iteration in any case.
+    // (The first barrier of the loop orders these reads before the dual step's sB writes.)
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int e = tid + NT * j;
+        const unsigned f = fl[j];
+        if (e >= E || !(f & F_ACT)) continue;
+        unsigned g = f;
+        if (sB[(f & F_HE) ? e + 1 : e] != 0.0f) g |= F_ME;
+        if (sB[(f & F_HW) ? e - 1 : e] != 0.0f) g |= F_MW;
+        if (sB[(f & F_HS) ? e + EX : e] != 0.0f) g |= F_MS;
+        if (sB[(f & F_HN) ? e - EX : e] != 0.0f) g |= F_MN;
+        fl[j] = g;
+    }
     if (skip) {
 #pragma unroll
         for (int j = 0; j < S; ++j) ub[j] = vbx[j] = vby[j] = 0.0f;
