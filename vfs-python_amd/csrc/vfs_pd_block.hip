@@ -529,6 +529,9 @@ const Variant kVariants[] = {
     // per lane for four (52 + 12 = 64 by 20 + 12 = 32); four pixels per lane (52 x 52) spills
     VFS_PD_VARIANT(23, 23, 3, 1024), VFS_PD_VARIANT(55, 23, 3, 1024), VFS_PD_VARIANT(23, 7, 3, 512),
     VFS_PD_VARIANT(52, 20, 4, 1024),
+    // deeper exact fits for the latency-bound coarse levels: fewer launches per warp
+    // (17 + 15 = 14 + 18 = 8 + 24 = 32)
+    VFS_PD_VARIANT(17, 17, 5, 1024), VFS_PD_VARIANT(14, 14, 6, 1024), VFS_PD_VARIANT(8, 8, 8, 1024),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
