@@ -334,19 +334,21 @@ def run_adjoint(a):
 
     v = ctxs[0]["vfs"]
     res = {}
-    for solver in ("graph", "blocked"):
-        v.solver = solver
+    t_fwd = t_bwd = None
+    if not a.no_compare:  # --no-compare: only the timed steps (e.g. under a counter run)
+        for solver in ("graph", "blocked"):
+            v.solver = solver
+            for _ in range(a.warmup):
+                v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"])
+            res[solver] = clock(lambda: v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"]), max(a.steps, 3))
+            log(f"[bench] forward solve ({solver}): {1e3 * res[solver]:.2f} ms")
+        adj = adjs[0]
         for _ in range(a.warmup):
-            v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"])
-        res[solver] = clock(lambda: v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"]), max(a.steps, 3))
-        log(f"[bench] forward solve ({solver}): {1e3 * res[solver]:.2f} ms")
-    adj = adjs[0]
-    for _ in range(a.warmup):
-        adj.forward(refs0[0], ctxs[0]["img_other"])
-        adj.backward()
-    t_fwd = clock(lambda: adj.forward(refs0[0], ctxs[0]["img_other"]), max(a.steps, 3))
-    t_bwd = clock(adj.backward, max(a.steps, 3))
-    log(f"[bench] adjoint forward (with saves) {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms")
+            adj.forward(refs0[0], ctxs[0]["img_other"])
+            adj.backward()
+        t_fwd = clock(lambda: adj.forward(refs0[0], ctxs[0]["img_other"]), max(a.steps, 3))
+        t_bwd = clock(adj.backward, max(a.steps, 3))
+        log(f"[bench] adjoint forward (with saves) {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms")
 
     losses, grads = [], []
 
@@ -380,10 +382,11 @@ def run_adjoint(a):
                    "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
                    "checkpoint": w["checkpoint"], "pairs": n_pairs},
         "forward_solve_ms": {k: round(1e3 * t, 3) for k, t in res.items()},
-        "adjoint_forward_ms": round(1e3 * t_fwd, 3),
-        "adjoint_backward_ms": round(1e3 * t_bwd, 3),
-        "backward_vs_plain_forward": round(t_bwd / res["graph"], 3),
-        "gradient_vs_plain_forward": round((t_fwd + t_bwd) / res["graph"], 3),
+        "adjoint_forward_ms": None if t_fwd is None else round(1e3 * t_fwd, 3),
+        "adjoint_backward_ms": None if t_bwd is None else round(1e3 * t_bwd, 3),
+        "backward_vs_plain_forward": None if t_bwd is None else round(t_bwd / res["graph"], 3),
+        "gradient_vs_plain_forward": None if t_bwd is None else round((t_fwd + t_bwd) / res["graph"], 3),
+        "gradient_vs_blocked_forward": None if t_bwd is None else round((t_fwd + t_bwd) / res["blocked"], 3),
         "mean_loss": round(float(np.mean(losses)), 6),
         "mean_gradient": {k: float(x) for k, x in zip(A.PARAMS, gmean)},
         "roofline": None,

@@ -74,7 +74,10 @@ struct AdjWs {
     // second copies: a fused launch reads one and writes the other (its ring reads
     // a neighbouring tile's values, which that tile's launch also rewrites)
     float *gp2, *gq2, *qmsg2, *tmsg2;
-    double* gpart;
+    // second copies of the state the blocked sweep also reads across tiles
+    float *gu2, *gub2, *gvb2;
+    double* gpart;    // fused sweep: (lambda, alpha0, alpha1) per 32 x 8 block
+    double* gpart_b;  // blocked sweep: the same per tile
     float* tape;
 };
 
@@ -86,17 +89,21 @@ inline int64_t fused_blocks(int H, int W)
     return static_cast<int64_t>((W + FT_X - 1) / FT_X) * ((H + FT_Y - 1) / FT_Y);
 }
 
-inline int64_t adj_ws_floats(int64_t n, int64_t nblocks)
+// Upper bound of blocked-sweep tiles (every reverse tile is at least 8 x 8).
+inline int64_t rev_tiles_max(int H, int W) { return static_cast<int64_t>((W + 7) / 8) * ((H + 7) / 8); }
+
+inline int64_t adj_ws_floats(int64_t n, int64_t nblocks, int64_t ntiles)
 {
     // tp 2, gp 2, gq 4, um 3, vm 6, qmsg 4, tmsg 4, gabc 3, geta 3, gderiv 2, gvbB 2, gubB 1, guf 1, gtv 2,
-    // gow 1, guth 1, gvb_fin 2, gIs 1, pos 1, and the second copies gp 2, gq 4, qmsg 4, tmsg 4
-    const int ch[] = {2, 2, 4, 3, 6, 4, 4, 3, 3, 2, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 4, 4, 4};
+    // gow 1, guth 1, gvb_fin 2, gIs 1, pos 1, the second copies gp 2, gq 4, qmsg 4, tmsg 4, gu 1, gubB 1, gvbB 2
+    const int ch[] = {2, 2, 4, 3, 6, 4, 4, 3, 3, 2, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 4, 4, 4, 1, 1, 2};
     int64_t t = 0;
     for (int c : ch) t += al(c * n);
-    return t + al(2 * 3 * nblocks);  // + (lambda, alpha0, alpha1) per block in double
+    // + (lambda, alpha0, alpha1) per block and per tile in double
+    return t + al(2 * 3 * nblocks) + al(2 * 3 * ntiles);
 }
 
-inline AdjWs adj_ws_at(float* base, int64_t n, int64_t nblocks)
+inline AdjWs adj_ws_at(float* base, int64_t n, int64_t nblocks, int64_t ntiles)
 {
     AdjWs a;
     float* c = base;
@@ -128,8 +135,13 @@ inline AdjWs adj_ws_at(float* base, int64_t n, int64_t nblocks)
     a.gq2 = take(4);
     a.qmsg2 = take(4);
     a.tmsg2 = take(4);
+    a.gu2 = take(1);
+    a.gub2 = take(1);
+    a.gvb2 = take(2);
     a.gpart = reinterpret_cast<double*>(c);
     c += al(2 * 3 * nblocks);
+    a.gpart_b = reinterpret_cast<double*>(c);
+    c += al(2 * 3 * ntiles);
     a.tape = c;
     return a;
 }
@@ -551,6 +563,466 @@ int launch_fused_rows(int rows, int kind, int nblocks, hipStream_t s, const Fuse
     return launch_fused<1>(kind, nblocks, s, fa);
 }
 
+// ---------------------------------------------------------------------------
+// Temporally blocked reverse sweep: up to KR consecutive units F(i) = dual^T(i) +
+// primal^T / threshold^T(i - 1) per launch, on a tile extended by KR pixels west /
+// north and 2 KR east / south.
+//
+// Per unit the region on which the adjoint state is exact shrinks by one pixel on
+// the east / south side (dual^T reads the E and S messages of the previous unit) and
+// by one more on every side (primal^T gathers the dual^T messages of x +- 1, y +- 1),
+// so after KR units the tile itself is exact.  The adjoint state (gu, g u_bar,
+// g v_bar, gp, gq) and the accumulators of abc, eta and the derivatives stay in
+// registers; the messages and, per unit, the neighbours' forward state go through LDS
+// (each pixel loads its own state from the tape).  Per-pixel arithmetic is the
+// fused kernel's (vfs_adj_pixel.h), in the same order, so the tile's values equal
+// the fused sweep's bit for bit; only the per-tile (lambda, alpha0, alpha1) sums are
+// grouped differently.
+// ---------------------------------------------------------------------------
+
+constexpr int KRMAX = 4;
+
+struct BlockRevArgs {
+    // adjoint state read (one set) and written (the other): every pixel of the level
+    const float* gu_in;
+    const float* gub_in;
+    const float2* gvb_in;
+    const float2* gp_in;
+    const float4* gq_in;
+    const float2* qm_in;
+    const float* tm_in;
+    float* gu_out;
+    float* gub_out;
+    float2* gvb_out;
+    float2* gp_out;
+    float4* gq_out;
+    float2* qm_out;
+    float* tm_out;
+    // accumulators, read and written at the tile's own pixels inside the mask
+    float* gabc;
+    float* geta;
+    float2* gderiv;
+    double* gpart;  // (lambda, alpha0, alpha1) per tile
+    const float* abc;
+    const float* eta;
+    const float2* deriv;
+    const uint8_t* mask;
+    // unit u reverses iteration i = i0 - u: forward states S_i and S_{i-1}
+    const float2* p1[KRMAX];
+    const float4* q1[KRMAX];
+    const float* ub1[KRMAX];
+    const float2* vb1[KRMAX];
+    const float* u2[KRMAX];
+    const float* ub2[KRMAX];
+    float sigma[KRMAX], tau_pl[KRMAX], mu_pl[KRMAX];
+    int n_units;
+    int H, W, tiles_x, tiles_y;
+    float alpha0, alpha1, eta_p, eta_q, lambda;
+};
+
+enum : unsigned {
+    RB_IMG = 1u,    // inside the image
+    RB_M = 2u,      // inside the mask
+    RB_ME = 4u,     // mask at the east neighbour (the pixel itself at the image border)
+    RB_MW = 8u,
+    RB_MS = 16u,
+    RB_MN = 32u,
+    RB_TXP = 64u,   // dual x-target: east neighbour (RB_TXM: west; neither: the pixel itself)
+    RB_TXM = 128u,
+    RB_TYP = 256u,  // dual y-target: south neighbour (RB_TYM: north)
+    RB_TYM = 512u,
+    RB_TILE = 1024u,
+};
+
+template <int TX, int TY, int KR, int NT>
+__global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
+{
+    constexpr int EX = TX + 3 * KR, EY = TY + 3 * KR, E = EX * EY, S = (E + NT - 1) / NT;
+    __shared__ float sQM[2 * E];     // primal^T messages (-g_S1, -g_S2), compact
+    __shared__ float sTM[E];         // threshold^T messages g_g
+    __shared__ float sDM[9 * E];     // dual^T messages (DualMsg um[3], vm[6])
+    // the unit's forward state S_i as its neighbours read it: T p (2), u_bar (1), v_bar (2),
+    // q (4); two buffers, alternating between units
+    __shared__ float sFS[2][9 * E];
+    __shared__ unsigned sFL[E];
+    __shared__ float s_red[3 * (NT / 64)];
+    const int tid = threadIdx.x;
+    const int H = a.H, W = a.W;
+    const int t = xcd_item(blockIdx.x, gridDim.x);
+    const int btx = t % a.tiles_x, bty = t / a.tiles_x;
+    const int ox = btx * TX - KR, oy = bty * TY - KR;
+
+    float gu[S], gub[S], gvx[S], gvy[S], gpx[S], gpy[S], gqx[S], gqy[S], gqz[S], gqw[S];
+    float ab[S][3], et[S][3], dd[S][2];   // accumulators (tile pixels)
+    float ca[S], cb[S], cc[S], e0[S], e1[S], e2[S], iu[S], iz[S];
+    int gi[S];
+    unsigned fl[S];
+    float gpl = 0.0f, gpa0 = 0.0f, gpa1 = 0.0f;
+
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int e = tid + NT * j;
+        const int ex = e % EX, ey = e / EX, gx = ox + ex, gy = oy + ey;
+        unsigned f = 0;
+        gi[j] = 0;
+        gu[j] = gub[j] = gvx[j] = gvy[j] = gpx[j] = gpy[j] = gqx[j] = gqy[j] = gqz[j] = gqw[j] = 0.0f;
+        ca[j] = cb[j] = cc[j] = e0[j] = e1[j] = e2[j] = iu[j] = iz[j] = 0.0f;
+        for (int c = 0; c < 3; ++c) ab[j][c] = et[j][c] = 0.0f;
+        dd[j][0] = dd[j][1] = 0.0f;
+        float qm0 = 0.0f, qm1 = 0.0f, tm0 = 0.0f;
+        if (e < E && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int idx = gy * W + gx;
+            gi[j] = idx;
+            f |= RB_IMG;
+            if (ex >= KR && ex < KR + TX && ey >= KR && ey < KR + TY) f |= RB_TILE;
+            if (a.mask[idx] != 0) {
+                f |= RB_M;
+                const bool mE = a.mask[gy * W + clamp_idx(gx + 1, W)] != 0;
+                const bool mS = a.mask[clamp_idx(gy + 1, H) * W + gx] != 0;
+                if (mE) f |= RB_ME;
+                if (a.mask[gy * W + clamp_idx(gx - 1, W)] != 0) f |= RB_MW;
+                if (mS) f |= RB_MS;
+                if (a.mask[clamp_idx(gy - 1, H) * W + gx] != 0) f |= RB_MN;
+                // the neighbour whose u_bar, v_bar this pixel's dual step read (dual_x_target)
+                if (mE ? gx + 1 < W : gx > 0) f |= mE ? RB_TXP : RB_TXM;
+                if (mS ? gy + 1 < H : gy > 0) f |= mS ? RB_TYP : RB_TYM;
+                ca[j] = a.abc[3 * idx];
+                cb[j] = a.abc[3 * idx + 1];
+                cc[j] = a.abc[3 * idx + 2];
+                e0[j] = a.eta[3 * idx];
+                e1[j] = a.eta[3 * idx + 1];
+                e2[j] = a.eta[3 * idx + 2];
+                const float2 d = a.deriv[idx];
+                iu[j] = d.x;
+                iz[j] = d.y;
+                if (f & RB_TILE) {
+                    for (int c = 0; c < 3; ++c) {
+                        ab[j][c] = a.gabc[3 * idx + c];
+                        et[j][c] = a.geta[3 * idx + c];
+                    }
+                    const float2 g = a.gderiv[idx];
+                    dd[j][0] = g.x;
+                    dd[j][1] = g.y;
+                }
+            }
+            gu[j] = a.gu_in[idx];
+            gub[j] = a.gub_in[idx];
+            const float2 gv = a.gvb_in[idx];
+            gvx[j] = gv.x;
+            gvy[j] = gv.y;
+            const float2 gp = a.gp_in[idx];
+            gpx[j] = gp.x;
+            gpy[j] = gp.y;
+            const float4 gq = a.gq_in[idx];
+            gqx[j] = gq.x;
+            gqy[j] = gq.y;
+            gqz[j] = gq.z;
+            gqw[j] = gq.w;
+            const float2 qm = a.qm_in[idx];
+            qm0 = qm.x;
+            qm1 = qm.y;
+            tm0 = a.tm_in[idx];
+        }
+        fl[j] = f;
+        if (e < E) {
+            sQM[e] = qm0;
+            sQM[E + e] = qm1;
+            sTM[e] = tm0;
+            sFL[e] = f;
+        }
+    }
+
+    for (int u = 0; u < a.n_units; ++u) {
+        float* fsb = sFS[u & 1];
+        // own forward state of this unit
+        float p1x[S], p1y[S], q1x[S], q1y[S], q1z[S], q1w[S], ub1[S], vb1x[S], vb1y[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int e = tid + NT * j;
+            p1x[j] = p1y[j] = q1x[j] = q1y[j] = q1z[j] = q1w[j] = ub1[j] = vb1x[j] = vb1y[j] = 0.0f;
+            float tx_ = 0.0f, ty_ = 0.0f;
+            if (fl[j] & RB_IMG) {
+                const int idx = gi[j];
+                ub1[j] = a.ub1[u][idx];
+                const float2 vb = a.vb1[u][idx];
+                vb1x[j] = vb.x;
+                vb1y[j] = vb.y;
+                if (fl[j] & RB_M) {
+                    const float2 p = a.p1[u][idx];
+                    const float4 q = a.q1[u][idx];
+                    p1x[j] = p.x;
+                    p1y[j] = p.y;
+                    q1x[j] = q.x;
+                    q1y[j] = q.y;
+                    q1z[j] = q.z;
+                    q1w[j] = q.w;
+                    tx_ = fmaf(ca[j], p.x, cc[j] * p.y);  // tp_at
+                    ty_ = fmaf(cc[j], p.x, cb[j] * p.y);
+                }
+            }
+            if (e < E) {
+                fsb[e] = tx_;
+                fsb[E + e] = ty_;
+                fsb[2 * E + e] = ub1[j];
+                fsb[3 * E + e] = vb1x[j];
+                fsb[4 * E + e] = vb1y[j];
+                fsb[5 * E + e] = q1x[j];
+                fsb[6 * E + e] = q1y[j];
+                fsb[7 * E + e] = q1z[j];
+                fsb[8 * E + e] = q1w[j];
+            }
+        }
+        __syncthreads();  // messages of the previous unit, T p of this one, flags
+        // dual^T of iteration i
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int e = tid + NT * j;
+            if (e >= E) continue;
+            const unsigned f = fl[j];
+            DualMsg m;
+            for (int c = 0; c < 3; ++c) m.um[c] = 0.0f;
+            for (int c = 0; c < 6; ++c) m.vm[c] = 0.0f;
+            float2 gpo = make_float2(0.0f, 0.0f);
+            float4 gqo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (f & RB_M) {
+                const int ex = e % EX, ey = e / EX, gx = ox + ex, gy = oy + ey;
+                // collect_qt_compact with the messages in LDS (a sender beyond the loaded
+                // region contributes nothing: such a pixel is inexact after this unit)
+                float4 gqn = make_float4(gqx[j], gqy[j], gqz[j], gqw[j]);
+                const float q0x = sQM[e], q0y = sQM[E + e], g0 = sTM[e];
+                float2 gtp = make_float2((f & RB_ME) ? g0 : 0.0f, (f & RB_MS) ? g0 : 0.0f);
+                if (gx == 0) {
+                    gqn.x += q0x;
+                    gqn.w += q0y;
+                    gtp.x += -g0;
+                }
+                if (gx + 1 < W && ex + 1 < EX) {
+                    gqn.x += sQM[e + 1];
+                    gqn.w += sQM[E + e + 1];
+                    gtp.x += -sTM[e + 1];
+                }
+                if (gy == 0) {
+                    gqn.y += q0y;
+                    gqn.z += q0x;
+                    gtp.y += -g0;
+                }
+                if (gy + 1 < H && ey + 1 < EY) {
+                    gqn.y += sQM[E + e + EX];
+                    gqn.z += sQM[e + EX];
+                    gtp.y += -sTM[e + EX];
+                }
+                const bool fx = (f & RB_ME) != 0, fy = (f & RB_MS) != 0;
+                // the neighbours read (the pixel itself at the image border, clamped; and at the
+                // region's own border, where the result is inexact after this unit in any case)
+                const int eE = (gx + 1 < W && ex + 1 < EX) ? e + 1 : e, eW = (gx > 0 && ex > 0) ? e - 1 : e;
+                const int eS = (gy + 1 < H && ey + 1 < EY) ? e + EX : e, eN = (gy > 0 && ey > 0) ? e - EX : e;
+                const int ix = fx ? eE : eW, iy = fy ? eS : eN;
+                // the per-pixel accumulators run at every pixel of the region (only the tile's
+                // are stored), the per-tile parameter sums at the tile's pixels only
+                float pa0 = 0.0f, pa1 = 0.0f;
+                dual_adj_pixel(fx, fy, ub1[j], make_float2(vb1x[j], vb1y[j]), fsb[2 * E + ix],
+                               make_float2(fsb[3 * E + ix], fsb[4 * E + ix]), fsb[2 * E + iy],
+                               make_float2(fsb[3 * E + iy], fsb[4 * E + iy]), ca[j], cb[j], cc[j],
+                               make_float2(p1x[j], p1y[j]),
+                               make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), a.sigma[u], a.alpha0, a.alpha1, a.eta_p,
+                               a.eta_q, make_float2(gpx[j], gpy[j]), gqn, gtp, gpo, gqo, m, ab[j], pa0, pa1);
+                if (f & RB_TILE) {
+                    gpa0 += pa0;
+                    gpa1 += pa1;
+                }
+            }
+            gpx[j] = gpo.x;
+            gpy[j] = gpo.y;
+            gqx[j] = gqo.x;
+            gqy[j] = gqo.y;
+            gqz[j] = gqo.z;
+            gqw[j] = gqo.w;
+            for (int c = 0; c < 3; ++c) sDM[c * E + e] = m.um[c];
+            for (int c = 0; c < 6; ++c) sDM[(3 + c) * E + e] = m.vm[c];
+        }
+        __syncthreads();  // dual^T messages
+        // primal^T and threshold^T of iteration i - 1
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int e = tid + NT * j;
+            if (e >= E) continue;
+            const unsigned f = fl[j];
+            if (!(f & RB_M)) {
+                gub[j] = gvx[j] = gvy[j] = gu[j] = 0.0f;
+                sQM[e] = sQM[E + e] = 0.0f;
+                sTM[e] = 0.0f;
+                continue;
+            }
+            const int ex = e % EX, ey = e / EX, gx = ox + ex, gy = oy + ey;
+            // gather_msgs from LDS: own, then the x senders X-1, X, X+1, the y senders Y-1, Y, Y+1
+            float g_ubo = gub[j];
+            float2 g_vbo = make_float2(gvx[j], gvy[j]);
+            g_ubo += sDM[e];
+            g_vbo.x += sDM[3 * E + e];
+            g_vbo.y += sDM[4 * E + e];
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xs = gx + dx, es = e + dx;
+                if (xs < 0 || xs >= W || ex + dx < 0 || ex + dx >= EX) continue;
+                const unsigned fs = sFL[es];
+                const unsigned want = dx < 0 ? RB_TXP : (dx > 0 ? RB_TXM : 0u);
+                if (!(fs & RB_M) || (fs & (RB_TXP | RB_TXM)) != want) continue;
+                g_ubo += sDM[E + es];
+                g_vbo.x += sDM[5 * E + es];
+                g_vbo.y += sDM[6 * E + es];
+            }
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int ys = gy + dy, es = e + dy * EX;
+                if (ys < 0 || ys >= H || ey + dy < 0 || ey + dy >= EY) continue;
+                const unsigned fs = sFL[es];
+                const unsigned want = dy < 0 ? RB_TYP : (dy > 0 ? RB_TYM : 0u);
+                if (!(fs & RB_M) || (fs & (RB_TYP | RB_TYM)) != want) continue;
+                g_ubo += sDM[2 * E + es];
+                g_vbo.x += sDM[7 * E + es];
+                g_vbo.y += sDM[8 * E + es];
+            }
+            Nbr n;
+            n.iE = n.iW = n.iS = n.iN = gi[j];
+            n.m = true;
+            n.mE = (f & RB_ME) != 0;
+            n.mW = (f & RB_MW) != 0;
+            n.mS = (f & RB_MS) != 0;
+            n.mN = (f & RB_MN) != 0;
+            const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // W / N neighbours (the pixel itself at the image border, or beyond the region)
+            const int eW = (gx > 0 && ex > 0) ? e - 1 : e, eN = (gy > 0 && ey > 0) ? e - EX : e;
+            float pa0 = 0.0f, pa1 = 0.0f, pl = 0.0f;
+            float2 gd = make_float2(dd[j][0], dd[j][1]);
+            float ubB, g_g;
+            float2 vbB, qmc;
+            float4 qm, tm;
+            float2 gp_ = make_float2(gpx[j], gpy[j]);
+            float4 gq_ = make_float4(gqx[j], gqy[j], gqz[j], gqw[j]);
+            float gu_ = gu[j];
+            float4 qW = make_float4(fsb[5 * E + eW], fsb[6 * E + eW], fsb[7 * E + eW], fsb[8 * E + eW]);
+            float4 qN = make_float4(fsb[5 * E + eN], fsb[6 * E + eN], fsb[7 * E + eN], fsb[8 * E + eN]);
+            if (!n.mW) qW = zero4;
+            if (!n.mN) qN = zero4;
+            primal_adj_pixel(n, make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), qW, qN, ca[j], cb[j], cc[j],
+                             make_float2(p1x[j], p1y[j]), e1[j],
+                             e2[j], a.tau_pl[u], a.mu_pl[u], a.alpha0, a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_,
+                             qm, ab[j], et[j][1], et[j][2], pa0, pa1, qmc);
+            gu_ = l1_adj_pixel(n, gu_, make_float2(fsb[e], fsb[E + e]), fsb[eW], fsb[E + eN], make_float2(iu[j], iz[j]),
+                               e0[j], a.u2[u][gi[j]], a.ub2[u][gi[j]], a.tau_pl[u], a.lambda, tm, ubB, et[j][0], gd,
+                               pl, g_g);
+            dd[j][0] = gd.x;
+            dd[j][1] = gd.y;
+            if (f & RB_TILE) {
+                gpa0 += pa0;
+                gpa1 += pa1;
+                gpl += pl;
+            }
+            gu[j] = gu_;
+            gub[j] = ubB;
+            gvx[j] = vbB.x;
+            gvy[j] = vbB.y;
+            gpx[j] = gp_.x;
+            gpy[j] = gp_.y;
+            gqx[j] = gq_.x;
+            gqy[j] = gq_.y;
+            gqz[j] = gq_.z;
+            gqw[j] = gq_.w;
+            sQM[e] = qmc.x;
+            sQM[E + e] = qmc.y;
+            sTM[e] = g_g;
+        }
+    }
+
+    // the tile's state and accumulators
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const unsigned f = fl[j];
+        if (!(f & RB_TILE)) continue;
+        const int e = tid + NT * j;
+        const int idx = gi[j];
+        a.gu_out[idx] = gu[j];
+        a.gub_out[idx] = gub[j];
+        a.gvb_out[idx] = make_float2(gvx[j], gvy[j]);
+        a.gp_out[idx] = make_float2(gpx[j], gpy[j]);
+        a.gq_out[idx] = make_float4(gqx[j], gqy[j], gqz[j], gqw[j]);
+        a.qm_out[idx] = make_float2(sQM[e], sQM[E + e]);
+        a.tm_out[idx] = sTM[e];
+        if (f & RB_M) {
+            for (int c = 0; c < 3; ++c) {
+                a.gabc[3 * idx + c] = ab[j][c];
+                a.geta[3 * idx + c] = et[j][c];
+            }
+            a.gderiv[idx] = make_float2(dd[j][0], dd[j][1]);
+        }
+    }
+    // tile sums of the lambda / alpha shares (wavefront shuffles, then the wavefronts)
+    float v[3] = {gpl, gpa0, gpa1};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if ((tid & 63) == 0)
+        for (int k = 0; k < 3; ++k) s_red[3 * (tid >> 6) + k] = v[k];
+    __syncthreads();
+    if (tid == 0) {
+        double* slot = a.gpart + 3 * (static_cast<int64_t>(bty) * a.tiles_x + btx);
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            for (int w = 0; w < NT / 64; ++w) acc += static_cast<double>(s_red[3 * w + k]);
+            slot[k] += acc;
+        }
+    }
+}
+
+// Adds each blocked tile's (lambda, alpha0, alpha1) sums to the per-pixel shares of
+// the tile's first pixel.
+__global__ void __launch_bounds__(256) k_fold_gpart_tiles(float* __restrict__ gpar, const double* __restrict__ gpart,
+                                                          int W, int tiles_x, int tile_x, int tile_y, int ntiles)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ntiles) return;
+    const int X = (b % tiles_x) * tile_x, Y = (b / tiles_x) * tile_y;
+    float* g = gpar + static_cast<int64_t>(GP_N) * (static_cast<int64_t>(Y) * W + X);
+    g[GP_LAMBDA] += static_cast<float>(gpart[3 * b]);
+    g[GP_ALPHA0] += static_cast<float>(gpart[3 * b + 1]);
+    g[GP_ALPHA1] += static_cast<float>(gpart[3 * b + 2]);
+}
+
+// Reverse variants (sweep modes): 20 + KR with 1024 threads on a 32 x 32 extended
+// tile, 30 + KR (one or two units) with 512 threads on 32 x 16.
+struct RevVariant {
+    int mode, tx, ty, kr, nt;
+};
+constexpr RevVariant kRevVariants[] = {
+    {21, 29, 29, 1, 1024}, {22, 26, 26, 2, 1024}, {23, 23, 23, 3, 1024}, {24, 20, 20, 4, 1024},
+    {31, 29, 13, 1, 512},  {32, 26, 10, 2, 512},
+};
+
+const RevVariant* find_rev_variant(int mode)
+{
+    for (const RevVariant& v : kRevVariants)
+        if (v.mode == mode) return &v;
+    return nullptr;
+}
+
+template <int TX, int TY, int KR, int NT>
+int launch_adj_block_t(hipStream_t s, int tiles, const BlockRevArgs& ba)
+{
+    hipLaunchKernelGGL((k_adj_block<TX, TY, KR, NT>), dim3(tiles), dim3(NT), 0, s, ba);
+    return hipGetLastError() == hipSuccess ? 0 : VFS_EINVAL;
+}
+
+int launch_adj_block(const RevVariant& v, hipStream_t s, int tiles, const BlockRevArgs& ba)
+{
+    switch (v.mode) {
+        case 21: return launch_adj_block_t<29, 29, 1, 1024>(s, tiles, ba);
+        case 22: return launch_adj_block_t<26, 26, 2, 1024>(s, tiles, ba);
+        case 23: return launch_adj_block_t<23, 23, 3, 1024>(s, tiles, ba);
+        case 24: return launch_adj_block_t<20, 20, 4, 1024>(s, tiles, ba);
+        case 31: return launch_adj_block_t<29, 13, 1, 512>(s, tiles, ba);
+        case 32: return launch_adj_block_t<26, 10, 2, 512>(s, tiles, ba);
+        default: return VFS_EINVAL;
+    }
+}
+
 #define VFS_TRY(expr)             \
     do {                          \
         int rc_ = (expr);         \
@@ -605,7 +1077,8 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt)
 {
     if (H <= 0 || W <= 0 || ckpt < 1) return 0;
     const int64_t n = static_cast<int64_t>(H) * W;
-    return static_cast<size_t>(adj_ws_floats(n, fused_blocks(H, W)) + static_cast<int64_t>(ckpt) * state_floats(n)) *
+    return static_cast<size_t>(adj_ws_floats(n, fused_blocks(H, W), rev_tiles_max(H, W)) +
+                               static_cast<int64_t>(ckpt) * state_floats(n)) *
            sizeof(float);
 }
 
@@ -625,7 +1098,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
     if (fk > 0) {
         // blocked path: k-iteration launches between two state sets, with a launch
         // boundary at every checkpoint; set 1 is the first tape slot of the workspace
-        const AdjWs A = adj_ws_at(workspace, n, fused_blocks(H, W));
+        const AdjWs A = adj_ws_at(workspace, n, fused_blocks(H, W), rev_tiles_max(H, W));
         const State t1 = state_at(A.tape, n);
         float* su[2] = {lv->u0, t1.u};
         float* sub[2] = {lv->u_bar, t1.ub};
@@ -726,18 +1199,26 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     for (const void* p : ptrs)
         if (p == nullptr) return VFS_EINVAL;
     if (save == nullptr && n_warp > 0) return VFS_EINVAL;
-    if (mode != 0 && mode != 1 && mode != 11 && mode != 12 && mode != 14) return VFS_EINVAL;
+    const RevVariant* rv = find_rev_variant(mode);
+    if (mode != 0 && mode != 1 && mode != 11 && mode != 12 && mode != 14 && rv == nullptr) return VFS_EINVAL;
     const int H = lv->H, W = lv->W;
     const int64_t n = static_cast<int64_t>(H) * W;
     hipStream_t s = S(stream);
     float* sv = const_cast<float*>(save);
     const int nbx = (W + FT_X - 1) / FT_X, nby = (H + FT_Y - 1) / FT_Y;
     const int64_t nblocks = static_cast<int64_t>(nbx) * nby;
-    const AdjWs A = adj_ws_at(workspace, n, nblocks);
+    const AdjWs A = adj_ws_at(workspace, n, nblocks, rev_tiles_max(H, W));
     const int nck = n_ckpt(n_iter, ckpt);
     VFS_TRY(vfs_fill(stream, A.gabc, 0.0f, 3 * n));
     VFS_TRY(vfs_fill(stream, A.geta, 0.0f, 3 * n));
     VFS_TRY(vfs_fill(stream, reinterpret_cast<float*>(A.gpart), 0.0f, 2 * 3 * nblocks));
+    // blocked sweep tiles (rv): the tile grid and its (lambda, alpha0, alpha1) slots
+    const int rtiles_x = rv ? (W + rv->tx - 1) / rv->tx : 0, rtiles_y = rv ? (H + rv->ty - 1) / rv->ty : 0;
+    const int rtiles = rtiles_x * rtiles_y;
+    if (rv != nullptr) {
+        if (rtiles > rev_tiles_max(H, W)) return VFS_EINVAL;
+        VFS_TRY(vfs_fill(stream, reinterpret_cast<float*>(A.gpart_b), 0.0f, 2 * 3 * static_cast<int64_t>(rtiles)));
+    }
     FusedArgs fa;
     fa.gu = A.guth;
     fa.gubB = A.gubB;
@@ -758,7 +1239,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     fa.eta_p = prm->eta_p;
     fa.eta_q = prm->eta_q;
     fa.lambda = prm->lambda;
-    const int rows = mode == 0 ? fused_rows(n) : mode - 10;
+    const int rows = (mode == 0 || rv != nullptr) ? fused_rows(n) : mode - 10;
     const int tile_y = FT_Y * rows, fnby = (H + tile_y - 1) / tile_y, fblocks = nbx * fnby;
     fa.nbx = nbx;
     fa.nby = fnby;
@@ -871,17 +1352,94 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
             fa.mu_pl = mus[n_iter - 1];
             set_bufs();
             VFS_TRY(launch_fused_rows(rows, 0, fblocks, s, fa));
-            for (int i = n_iter - 1; i >= 1; --i) {
-                if (i - 1 < cs * ckpt) {
-                    --cs;
-                    VFS_TRY(recompute(cs));
+            if (rv == nullptr) {
+                for (int i = n_iter - 1; i >= 1; --i) {
+                    if (i - 1 < cs * ckpt) {
+                        --cs;
+                        VFS_TRY(recompute(cs));
+                    }
+                    set_states(state_of(cs, i), state_of(cs, i - 1));
+                    fa.sigma = sigmas[i];
+                    fa.tau_pl = taus[i - 1];
+                    fa.mu_pl = mus[i - 1];
+                    set_bufs();
+                    VFS_TRY(launch_fused_rows(rows, 1, fblocks, s, fa));
                 }
-                set_states(state_of(cs, i), state_of(cs, i - 1));
-                fa.sigma = sigmas[i];
-                fa.tau_pl = taus[i - 1];
-                fa.mu_pl = mus[i - 1];
-                set_bufs();
-                VFS_TRY(launch_fused_rows(rows, 1, fblocks, s, fa));
+            } else {
+                // F(i), F(i - 1), ... in launches of up to rv->kr units, each within one
+                // checkpoint segment; (gu, g u_bar, g v_bar) ping-pong like (gp, gq, messages)
+                float* gus[2] = {A.guth, A.gu2};
+                float* gubs[2] = {A.gubB, A.gub2};
+                float* gvbs[2] = {A.gvbB, A.gvb2};
+                float* gpi[2] = {A.gp, A.gp2};
+                float* gqi[2] = {A.gq, A.gq2};
+                float* qmi[2] = {A.qmsg, A.qmsg2};
+                float* tmi[2] = {A.tmsg, A.tmsg2};
+                int scur = 0;
+                int i = n_iter - 1;
+                while (i >= 1) {
+                    if (i - 1 < cs * ckpt) {
+                        --cs;
+                        VFS_TRY(recompute(cs));
+                    }
+                    int kc = rv->kr;
+                    if (kc > i - cs * ckpt) kc = i - cs * ckpt;
+                    BlockRevArgs ba;
+                    ba.gu_in = gus[scur];
+                    ba.gub_in = gubs[scur];
+                    ba.gvb_in = reinterpret_cast<const float2*>(gvbs[scur]);
+                    ba.gp_in = reinterpret_cast<const float2*>(gpi[cur]);
+                    ba.gq_in = reinterpret_cast<const float4*>(gqi[cur]);
+                    ba.qm_in = reinterpret_cast<const float2*>(qmi[cur]);
+                    ba.tm_in = tmi[cur];
+                    ba.gu_out = gus[1 - scur];
+                    ba.gub_out = gubs[1 - scur];
+                    ba.gvb_out = reinterpret_cast<float2*>(gvbs[1 - scur]);
+                    ba.gp_out = reinterpret_cast<float2*>(gpi[1 - cur]);
+                    ba.gq_out = reinterpret_cast<float4*>(gqi[1 - cur]);
+                    ba.qm_out = reinterpret_cast<float2*>(qmi[1 - cur]);
+                    ba.tm_out = tmi[1 - cur];
+                    ba.gabc = A.gabc;
+                    ba.geta = A.geta;
+                    ba.gderiv = reinterpret_cast<float2*>(A.gderiv);
+                    ba.gpart = A.gpart_b;
+                    ba.abc = lv->abc;
+                    ba.eta = lv->eta;
+                    ba.deriv = reinterpret_cast<const float2*>(lv->deriv);
+                    ba.mask = lv->mask;
+                    for (int u = 0; u < KRMAX; ++u) {
+                        const int iu = u < kc ? i - u : i;  // unused slots repeat the first unit
+                        const State si = state_of(cs, iu), sm = state_of(cs, iu - 1);
+                        ba.p1[u] = reinterpret_cast<const float2*>(si.p);
+                        ba.q1[u] = reinterpret_cast<const float4*>(si.q);
+                        ba.ub1[u] = si.ub;
+                        ba.vb1[u] = reinterpret_cast<const float2*>(si.vb);
+                        ba.u2[u] = sm.u;
+                        ba.ub2[u] = sm.ub;
+                        ba.sigma[u] = sigmas[iu];
+                        ba.tau_pl[u] = taus[iu - 1];
+                        ba.mu_pl[u] = mus[iu - 1];
+                    }
+                    ba.n_units = kc;
+                    ba.H = H;
+                    ba.W = W;
+                    ba.tiles_x = rtiles_x;
+                    ba.tiles_y = rtiles_y;
+                    ba.alpha0 = prm->alpha0;
+                    ba.alpha1 = prm->alpha1;
+                    ba.eta_p = prm->eta_p;
+                    ba.eta_q = prm->eta_q;
+                    ba.lambda = prm->lambda;
+                    VFS_TRY(launch_adj_block(*rv, s, rtiles, ba));
+                    cur = 1 - cur;
+                    scur = 1 - scur;
+                    i -= kc;
+                }
+                if (scur == 1) {
+                    VFS_TRY(vfs_copy(stream, A.guth, A.gu2, n));
+                    VFS_TRY(vfs_copy(stream, A.gubB, A.gub2, n));
+                    VFS_TRY(vfs_copy(stream, A.gvbB, A.gvb2, 2 * n));
+                }
             }
             const State s0 = state_at(ws.ckpt0, n);
             set_states(s0, s0);
@@ -947,6 +1505,11 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         hipLaunchKernelGGL(k_fold_gpart, dim3(nb1(fblocks)), dim3(256), 0, s, gpar, A.gpart, W, nbx, tile_y,
                            fblocks);
         if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        if (rv != nullptr) {
+            hipLaunchKernelGGL(k_fold_gpart_tiles, dim3(nb1(rtiles)), dim3(256), 0, s, gpar, A.gpart_b, W, rtiles_x,
+                               rv->tx, rv->ty, rtiles);
+            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        }
     }
     VFS_TRY(vfs_fill(stream, A.gIs, 0.0f, n));
     VFS_TRY(vfs_adj_tensor(stream, A.gIs, gpar, A.gabc, A.geta, lv->ref_smoothed, lv->mask, H, W, prm));

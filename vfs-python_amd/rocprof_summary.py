@@ -57,7 +57,7 @@ def stats(d, title, cmd, top=18):
         a[1] += us
         a[2] = min(a[2], us)
         a[3] = max(a[3], us)
-        if n.startswith(("k_pd_block", "k_adj_fused")):
+        if n.startswith(("k_pd_block", "k_adj_fused", "k_adj_block")):
             g = shape.setdefault((n, int(r.get("Grid_Size", r.get("Grid_Size_X", 0)))), [0, 0.0])
             g[0] += 1
             g[1] += us

@@ -47,7 +47,7 @@ def _e(shape, ch=None):
 
 class VFSDepthAdjoint:
     def __init__(self, param, mask, tvf, cvf, cam, t, z_gt, checkpoint=25, fused=True, rows=None, blocked=True,
-                 rec_tile="auto", workspace=None, reverse_warps=0):
+                 rec_tile="auto", workspace=None, reverse_warps=0, reverse_block=None):
         import vfs as V
         self.param = copy.copy(param)
         self.checkpoint = int(checkpoint)
@@ -57,6 +57,16 @@ class VFSDepthAdjoint:
         # reverse sweep: 1 = one kernel per step; 0 = fused, rows per thread by level size;
         # 10 + rows = fused with that many rows per thread
         self._mode = 1 if not self.fused else (0 if rows is None else 10 + rows)
+        # temporally blocked sweep: reverse_block = K units per launch on a 32 x 32 extended
+        # tile (1024 threads; sweep 20 + K), or (K, 512) for one or two units on 32 x 16
+        if reverse_block is not None:
+            if not self.fused or rows is not None:
+                raise ValueError("reverse_block replaces the fused sweep's middle launches; keep fused=True, rows=None")
+            modes = {1: 21, 2: 22, 3: 23, 4: 24, (1, 512): 31, (2, 512): 32}
+            key = reverse_block if isinstance(reverse_block, int) else tuple(reverse_block)
+            if key not in modes:
+                raise ValueError(f"reverse_block must be one of {list(modes)}")
+            self._mode = modes[key]
         # forward: blocked primal-dual launches (variant per level from the autotune table)
         # or the per-kernel steps; tape recompute: blocked launches that also write
         # their intermediate states, with the forward's variant ("auto") or a given
