@@ -291,7 +291,7 @@ def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt, rows):
     res = []
     for fused in (False, True):
         adj = A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=ckpt,
-                                fused=fused, rows=rows)
+                                fused=fused, rows=rows, reverse_block=None)
         loss = adj.forward(ref0, pr["img_other"])
         res.append((loss, adj.backward()))
     (l0, g0), (l1, g1) = res

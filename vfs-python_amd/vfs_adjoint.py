@@ -47,7 +47,7 @@ def _e(shape, ch=None):
 
 class VFSDepthAdjoint:
     def __init__(self, param, mask, tvf, cvf, cam, t, z_gt, checkpoint=25, fused=True, rows=None, blocked=True,
-                 rec_tile="auto", workspace=None, reverse_warps=0, reverse_block=None):
+                 rec_tile="auto", workspace=None, reverse_warps=0, reverse_block="auto"):
         import vfs as V
         self.param = copy.copy(param)
         self.checkpoint = int(checkpoint)
@@ -57,16 +57,30 @@ class VFSDepthAdjoint:
         # reverse sweep: 1 = one kernel per step; 0 = fused, rows per thread by level size;
         # 10 + rows = fused with that many rows per thread
         self._mode = 1 if not self.fused else (0 if rows is None else 10 + rows)
+        self._auto_block = False
         # temporally blocked sweep: reverse_block = K units per launch on a 32 x 32 extended
-        # tile (1024 threads; sweep 20 + K), or (K, 512) for one or two units on 32 x 16
-        if reverse_block is not None:
+        # tile (1024 threads; sweep 20 + K), or (K, 512) for one or two units on 32 x 16;
+        # a dict {level: choice} sets it per pyramid level (missing levels: fused); "auto"
+        # picks by level size (three units at >= 400 k pixels, two below, and two on the
+        # 512-thread tile under 100 k: measured at the 800 x 800 gradient workload)
+        self._level_mode = {}
+        if reverse_block == "auto":
+            reverse_block = None if (not self.fused or rows is not None) else "auto"
+        if reverse_block == "auto":
+            self._auto_block = True
+        elif reverse_block is not None:
             if not self.fused or rows is not None:
                 raise ValueError("reverse_block replaces the fused sweep's middle launches; keep fused=True, rows=None")
             modes = {1: 21, 2: 22, 3: 23, 4: 24, (1, 512): 31, (2, 512): 32}
-            key = reverse_block if isinstance(reverse_block, int) else tuple(reverse_block)
-            if key not in modes:
-                raise ValueError(f"reverse_block must be one of {list(modes)}")
-            self._mode = modes[key]
+            per = reverse_block if isinstance(reverse_block, dict) else None
+            for lvl, rb in (per.items() if per is not None else [(None, reverse_block)]):
+                key = rb if isinstance(rb, int) else tuple(rb)
+                if key not in modes:
+                    raise ValueError(f"reverse_block must be one of {list(modes)}")
+                if lvl is None:
+                    self._mode = modes[key]
+                else:
+                    self._level_mode[int(lvl)] = modes[key]
         # forward: blocked primal-dual launches (variant per level from the autotune table)
         # or the per-kernel steps; tape recompute: blocked launches that also write
         # their intermediate states, with the forward's variant ("auto") or a given
@@ -119,7 +133,11 @@ class VFSDepthAdjoint:
         if self.rec_tile is not None:
             rec = best if self.rec_tile == "auto" else autotune.variant(self.rec_tile)
             t.rec_tile_x, t.rec_tile_y, t.rec_k, t.rec_threads = rec
-        t.sweep = self._mode
+        if self._auto_block:
+            npx = hh * ww
+            t.sweep = 23 if npx >= 400_000 else (22 if npx >= 100_000 else 32)
+        else:
+            t.sweep = self._level_mode.get(k, self._mode)
         t.reverse_warps = self.reverse_warps
         return t
 
