@@ -340,3 +340,58 @@ def test_blocked_sweep_matches_fused_sweep(T, shape, ckpt, rb):
     d = float((g1["ref0"] - g0["ref0"]).abs().max())
     print("ref0 max diff", d, "max", float(g0["ref0"].abs().max()))
     assert d <= 1e-5 * float(g0["ref0"].abs().max())
+
+
+def test_adjoint_at_default_parameters_matches_autograd(T):
+    """The gradient at the reference's default parameters and pyramid ratio (fScale 1.2,
+    lambda 5, alpha0 17, alpha1 1.2, beta 9, gamma 0.85) on a non-square 44 x 52 pair with
+    four levels down to 25 x 29, so the up- and downsampling adjoints run at a general
+    ratio: against float64 autograd through the unrolled restatement.  With these
+    parameters float32 arithmetic alone moves the parameter gradients by up to 6 % (the
+    same restatement under float32 autograd: lambda -0.0655 against -0.0688), so each is
+    held to max(2e-2 relative, 3x that float32 deviation); the image gradient to cosine
+    >= 0.999."""
+    import synthetic as S
+    import vfs_adjoint as A
+    import vfs_params as P
+    hh, ww, levels, n_warp, n_iter = 44, 52, 4, 2, 6
+    pr = S.make_pair(hh, ww, 6, t=(0.2, 0.0, 0.02), rot_deg=1.0, n_levels=levels, level_ratio=1.2)
+    tvf, cvf = O.get_vector_fields(O.get_grid_zero((hh, ww)), pr["cam"], pr["cam"], pr["R"], pr["t"], 1)
+    tvf, cvf = S.sanitize_field(tvf), cvf.astype(np.float32)
+    prm = P.vfs_params(pr["img_ref"])
+    prm.nLevel, prm.nWarpIters, prm.nSolverIters = levels, n_warp, n_iter
+    assert prm.fScale == 1.2
+    theta = {k: float(getattr(prm, k)) for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma")}
+    assert theta == dict(Lambda=5.0, alpha0=17.0, alpha1=1.2, beta=9.0, gamma=0.85)
+    adj = A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"], checkpoint=4)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    loss = adj.forward(ref0, pr["img_other"])
+    g = adj.backward()
+    sizes = O.compute_pyramid_sizes(hh, ww, levels, 1.2)
+    assert sizes[-1] == (25, 29)
+    masks = [torch.as_tensor(m > 0) for m in O.create_mask_pyramid(pr["mask"], sizes)]
+    trs = [torch.as_tensor(t) for t in O.create_vector_field_pyramid(tvf, sizes)]
+    r0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"])).requires_grad_(True)
+    ow = torch.as_tensor(O.warping_image(O.cast_and_scale(pr["img_other"]), cvf))
+    th = [torch.tensor(theta[k], dtype=torch.float64, requires_grad=True) for k in
+          ("Lambda", "alpha0", "alpha1", "beta", "gamma")]
+    flow = R.solve_stereo(r0, ow, masks, trs, sizes, *th, n_warp, n_iter, f_scale=1.2)
+    lt = R.depth_loss(flow, pr["cam"], torch.as_tensor(pr["t"].reshape(3)), masks[0], torch.as_tensor(pr["z_gt"]))
+    lt.backward()
+    # the same restatement under float32 autograd
+    f = torch.float32
+    th32 = [torch.tensor(theta[k], dtype=f, requires_grad=True) for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma")]
+    flow32 = R.solve_stereo(torch.as_tensor(O.cast_and_scale(pr["img_ref"])).to(f), ow.to(f), masks,
+                            [t.to(f) for t in trs], sizes, *th32, n_warp, n_iter, f_scale=1.2)
+    R.depth_loss(flow32, pr["cam"], torch.as_tensor(pr["t"].reshape(3)).to(f), masks[0],
+                 torch.as_tensor(pr["z_gt"]).to(f)).backward()
+    print("loss hip", loss, "torch", float(lt))
+    assert abs(loss - float(lt)) <= 1e-3 * abs(float(lt))
+    for i, k in enumerate(("Lambda", "alpha0", "alpha1", "beta", "gamma")):
+        ref, r32 = th[i].grad.item(), th32[i].grad.item()
+        print(k, g[k], ref, "fp32 restatement", r32)
+        assert abs(g[k] - ref) <= max(2e-2 * abs(ref), 3 * abs(r32 - ref)) + 1e-6, k
+    gr, rr = g["ref0"].cpu().numpy().ravel(), r0.grad.numpy().ravel()
+    cos = float(np.dot(gr, rr) / (np.linalg.norm(gr) * np.linalg.norm(rr)))
+    print("ref0 cosine", cos, "rel", np.linalg.norm(gr - rr) / np.linalg.norm(rr))
+    assert cos >= 0.999
