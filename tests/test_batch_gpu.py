@@ -78,3 +78,46 @@ def test_batch_equals_single_pair_solves(T):
         st = flow_diff_stats(f, fo, c["pair"]["mask"])
         print((h, w), "batched vs oracle:", st)
         assert st["median"] <= 1e-4
+
+
+def test_batch_at_default_parameters_equals_single_solves(T):
+    """The reference's default parameters and ratio (11 levels at fScale 1.2, lambda 5,
+    alpha0 17, alpha1 1.2, beta 9, gamma 0.85; 2 warps x 8 iterations per level) on three
+    pairs of uneven odd sizes: every level of every pair has its own size and the batched
+    launches mix them; with the autotuned tiles of each launch's total pixel count, each
+    flow equals its single-pair solve with the same tiles bit for bit."""
+    import copy
+    import synthetic as S
+    import vfs as V
+    import vfs_batch as B
+    import vfs_params as P
+    sizes = [(101, 133), (120, 136), (87, 97)]
+    pairs = []
+    for i, (h, w) in enumerate(sizes):
+        pr = S.make_pair(h, w, seed=200 + i, t=(0.2, 0.0, 0.02), rot_deg=1.5, rot_random=True, n_levels=3,
+                         level_ratio=1.2)
+        tvf, cvf = O.get_vector_fields(O.get_grid_zero((h, w)), pr["cam"], pr["cam"], pr["R"], pr["t"], 1)
+        pairs.append(dict(pair=pr, tvf=S.sanitize_field(tvf), cvf=cvf.astype(np.float32)))
+    prm = P.vfs_params()
+    assert (prm.nLevel, prm.fScale) == (11, 1.2)
+    prm.nWarpIters, prm.nSolverIters = 2, 8
+    batch = B.VFSBatch(prm, sizes)
+    for i, c in enumerate(pairs):
+        batch.setup_pair(i, c["pair"]["mask"], c["tvf"], c["cvf"])
+    flows = [f.cpu().numpy() for f in batch.solve([c["pair"]["img_ref"] for c in pairs],
+                                                  [c["pair"]["img_other"] for c in pairs])]
+    tiles = batch._tiles()
+    for (h, w), c, f in zip(sizes, pairs, flows):
+        p = copy.copy(prm)
+        p.height, p.width = h, w
+        v = V.VFS(p)
+        v.solver = "blocked"
+        v.pd_tiles = {lvl: tuple(t) for lvl, t in enumerate(tiles)}
+        v.setup_module()
+        v.compute_pyramid_sizes()
+        assert len(v.sizes) == 11
+        v.create_mask_pyramid(c["pair"]["mask"])
+        v.create_vector_field_pyramid(c["tvf"], c["cvf"])
+        single = v.solve_stereo(c["pair"]["img_ref"], c["pair"]["img_other"]).cpu().numpy()
+        np.testing.assert_array_equal(f, single)
+        assert np.isfinite(f).all() and np.abs(f).max() > 0
