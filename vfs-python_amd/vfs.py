@@ -12,18 +12,19 @@ source at the first ``setup_module``, Python/vfs.py:38-74).
 ``solve_stereo`` runs the reference's loop (Python/vfs.py:460-533).  The
 attribute ``solver`` selects how it is driven:
 
-* ``"graph"`` (default): the whole solve -- pyramids, every level's warp loop
-  with the reference's kernel decomposition, upsampling, final masking -- is
-  recorded once into a HIP graph and replayed per call;
+* ``"blocked"`` (default) / ``"blocked_eager"``: the primal-dual loop temporally
+  blocked (k iterations per launch on LDS-exchanged register tiles, see
+  csrc/vfs_pd_block.hip) and the per-warp kernels fused, recorded once into a HIP
+  graph and replayed per call, or enqueued per call.  The tile shape and k per
+  pyramid level come from ``pd_tiles`` (a dict {level: (tile_x, tile_y, k[, threads])})
+  or from the autotuned table (autotune.py).  The result equals the per-kernel
+  solve's bit for bit;
+* ``"graph"``: the whole solve with the reference's kernel decomposition --
+  pyramids, every level's warp loop, upsampling, final masking -- recorded once
+  into a HIP graph and replayed per call;
 * ``"eager"``: the same launches enqueued from C++ on each call;
 * ``"python"``: the reference's host loop literally, one Python call per
-  kernel through the per-kernel methods below;
-* ``"blocked"`` / ``"blocked_eager"``: the same solve with the primal-dual loop
-  temporally blocked (k iterations per launch on LDS-exchanged register tiles,
-  see csrc/vfs_pd_block.hip) and the per-warp kernels fused, recorded or
-  enqueued per call.  The tile shape and k per pyramid level come from
-  ``pd_tiles`` (a dict {level: (tile_x, tile_y, k[, threads])}) or from the autotuned
-  table (autotune.py).
+  kernel through the per-kernel methods below.
 
 A recorded solve holds at most ``GRAPH_MAX_LAUNCHES`` kernel launches; a
 larger one (the per-kernel solve at the default 11 levels x 100 warps x 100
@@ -72,7 +73,7 @@ class VFS:
     def __init__(self, param):
         self.param = param
         self.gpu_module = None
-        self.solver = "graph"
+        self.solver = "blocked"
         # solver="python" only: per warp, the RMS over the mask of (thresholded u -
         # warp-start u), as (level, warp, rms) in self.residuals after a solve
         self.track_residuals = False
