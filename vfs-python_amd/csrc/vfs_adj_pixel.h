@@ -126,8 +126,9 @@ __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4
     }
     const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
     const bool d1 = !(fabsf(e1) < kEps), d2 = !(fabsf(e2) < kEps);
-    const float t1 = tau * recip_eta(e1, kEps);
-    const float t2 = tau * recip_eta(e2, kEps);
+    const float r1 = recip_eta(e1, kEps), r2 = recip_eta(e2, kEps);
+    const float t1 = tau * r1;
+    const float t2 = tau * r2;
     const float S1 = gQ.x + gQ.z, S2 = gQ.w + gQ.y;
     const float A1 = fmaf(alpha1, tu.x, alpha0 * S1), A2 = fmaf(alpha1, tu.y, alpha0 * S2);
     // ub_out = (1 + mu) u - mu ub ; vb_out = (1 + mu) v - mu vb ; v = t A + vb
@@ -140,8 +141,9 @@ __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4
     const float g_t1 = g_vx * A1, g_t2 = g_vy * A2;
     gpa1 += g_A1 * tu.x + g_A2 * tu.y;
     gpa0 += g_A1 * S1 + g_A2 * S2;
-    if (d1) geta1 += -g_t1 * tau / (e1 * e1);
-    if (d2) geta2 += -g_t2 * tau / (e2 * e2);
+    // d t / d eta = -tau / eta^2 = -t / eta
+    if (d1) geta1 += -g_t1 * t1 * r1;
+    if (d2) geta2 += -g_t2 * t2 * r2;
     const float2 g_tu = make_float2(g_A1 * alpha1, g_A2 * alpha1);
     gp.x += a * g_tu.x + c * g_tu.y;
     gp.y += c * g_tu.x + b * g_tu.y;
@@ -208,7 +210,8 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
     const float gy = (n.mS && n.mN) ? tc.y - tpNy : (!n.mS ? -tpNy : tc.y);
     const float dIu = d.x, dIz = d.y;
     const bool de = !(fabsf(e0) < kEps);
-    const float te = tau * recip_eta(e0, kEps);
+    const float r0 = recip_eta(e0, kEps);
+    const float te = tau * r0;
     const float ut = ubc + te * (gx + gy);
     const float rho = dIu * (ut - uc) + dIz;
     const float bound = lambda * te * dIu * dIu;
@@ -240,7 +243,7 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
     gubB += g_ut;
     g_te += g_ut * (gx + gy);
     const float g_g = g_ut * te;
-    if (de) geta0 += -g_te * tau / (e0 * e0);
+    if (de) geta0 += -g_te * te * r0;
     gderiv.x += g_dIu;
     gderiv.y += g_dIz;
     gplam += g_lam;
@@ -318,6 +321,20 @@ __device__ __forceinline__ void collect_qt_compact(const float2* __restrict__ qm
     if (Y + 1 < H) gtp.y += -tm[idx + W];
 }
 
+// Step sizes of the dual ascent of one iteration and their derivatives in alpha1 and
+// alpha0: sp = dual_scale(alpha1, sigma, eta_p) as the forward formed it, dsp = d sp / d
+// alpha1 = sigma / eta_p, and likewise sq, dsq.  Formed once per launch on the host.
+struct DualSteps {
+    float sp, sq, dsp, dsq;
+};
+
+__host__ __device__ __forceinline__ DualSteps dual_steps(float sigma, float alpha0, float alpha1, float eta_p,
+                                                         float eta_q)
+{
+    return DualSteps{dual_scale(alpha1, sigma, eta_p), dual_scale(alpha0, sigma, eta_q), sigma / eta_p,
+                     sigma / eta_q};
+}
+
 // Reverse of updateDualVariables (Python/vfs.cu:326-409) at a masked-in pixel.
 // gpn, gqn, gtp: complete adjoints of p_t, q_t, tp_t.  State: p_{t-1}, q_{t-1} at the
 // pixel; u_bar_{t-1}, v_bar_{t-1} at the pixel and its x / y targets ix, iy (E / S
@@ -325,10 +342,10 @@ __device__ __forceinline__ void collect_qt_compact(const float2* __restrict__ qm
 // Sets gp, gq := adjoints of p_{t-1}, q_{t-1} and the messages; adds to gabc and
 // the alpha shares.
 __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float2 vc, float uX, float2 vX, float uY,
-                                               float2 vY, float a, float b, float c, float2 pc, float4 qc, float sigma,
-                                               float alpha0, float alpha1, float eta_p, float eta_q, float2 gpn,
-                                               float4 gqn, float2 gtp, float2& gp_out, float4& gq_out, DualMsg& msg,
-                                               float (&gabc)[3], float& gpa0, float& gpa1)
+                                               float2 vY, float a, float b, float c, float2 pc, float4 qc,
+                                               const DualSteps& st, float2 gpn, float4 gqn, float2 gtp,
+                                               float2& gp_out, float4& gq_out, DualMsg& msg, float (&gabc)[3],
+                                               float& gpa0, float& gpa1)
 {
     const float sx = fx ? 1.0f : -1.0f, sy = fy ? 1.0f : -1.0f;
     float2 gU;
@@ -351,7 +368,7 @@ __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float
         gV.y = vc.y - vY.y;
         gV.z = vc.x - vY.x;
     }
-    const float sp = dual_scale(alpha1, sigma, eta_p), sq = dual_scale(alpha0, sigma, eta_q);
+    const float sp = st.sp, sq = st.sq;
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
     const float2 up = make_float2(fmaf(sp, tu.x, pc.x), fmaf(sp, tu.y, pc.y));
     // the forward projections (dual_pixel): scale r = 1 / |x| where |x|^2 > 1
@@ -385,8 +402,8 @@ __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float
     gq_out = guq;
     const float g_sp = gup.x * tu.x + gup.y * tu.y;
     const float g_sq = guq.x * gV.x + guq.y * gV.y + guq.z * gV.z + guq.w * gV.w;
-    gpa1 += g_sp * sigma / eta_p;
-    gpa0 += g_sq * sigma / eta_q;
+    gpa1 += g_sp * st.dsp;
+    gpa0 += g_sq * st.dsq;
     const float2 gtu = make_float2(sp * gup.x, sp * gup.y);
     ga += gtu.x * gU.x;
     gb += gtu.y * gU.y;

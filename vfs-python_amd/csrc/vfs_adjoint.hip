@@ -150,8 +150,7 @@ __global__ void __launch_bounds__(256) k_adj_dual(float2* __restrict__ gp, float
                                                   const float* __restrict__ u_bar_prev,
                                                   const float2* __restrict__ v_bar_prev,
                                                   const float* __restrict__ abc, const uint8_t* __restrict__ mask,
-                                                  int H, int W, float sigma, float alpha0, float alpha1, float eta_p,
-                                                  float eta_q)
+                                                  int H, int W, DualSteps st)
 {
     VFS_ADJ_PIXEL(H, W)
     const Nbr n = nbr(mask, X, Y, H, W);
@@ -174,8 +173,8 @@ __global__ void __launch_bounds__(256) k_adj_dual(float2* __restrict__ gp, float
     float* gpp = gpar + GP_N * idx;
     float pa0 = gpp[GP_ALPHA0], pa1 = gpp[GP_ALPHA1];
     dual_adj_pixel(fx, fy, u_bar_prev[idx], v_bar_prev[idx], u_bar_prev[ix], v_bar_prev[ix], u_bar_prev[iy],
-                   v_bar_prev[iy], abc[3 * idx], abc[3 * idx + 1], abc[3 * idx + 2], p_prev[idx], q_prev[idx], sigma,
-                   alpha0, alpha1, eta_p, eta_q, gp[idx], gqn, gtp, gpo, gqo, msg, ga, pa0, pa1);
+                   v_bar_prev[iy], abc[3 * idx], abc[3 * idx + 1], abc[3 * idx + 2], p_prev[idx], q_prev[idx], st,
+                   gp[idx], gqn, gtp, gpo, gqo, msg, ga, pa0, pa1);
     gp[idx] = gpo;
     gq[idx] = gqo;
     gpp[GP_ALPHA1] = pa1;
@@ -654,8 +653,8 @@ int vfs_adj_pd_iteration(void* stream, const vfs_adj_pd* a, const vfs_params* pr
                        reinterpret_cast<float4*>(a->gq), reinterpret_cast<const float4*>(a->qmsg),
                        reinterpret_cast<const float4*>(a->tmsg), a->um, a->vm, a->gabc, a->gpar,
                        reinterpret_cast<const float2*>(a->p_prev), reinterpret_cast<const float4*>(a->q_prev),
-                       a->u_bar_prev, reinterpret_cast<const float2*>(a->v_bar_prev), a->abc, a->mask, H, W, sigma,
-                       prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q);
+                       a->u_bar_prev, reinterpret_cast<const float2*>(a->v_bar_prev), a->abc, a->mask, H, W,
+                       dual_steps(sigma, prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q));
     VFS_LAUNCH_STATUS();
 }
 

@@ -321,7 +321,7 @@ struct FusedArgs {
     const uint8_t* mask;
     int H, W;
     int nbx, nby;         // tiles
-    float sigma;          // iteration i
+    DualSteps st;         // iteration i (dual_steps of its sigma)
     float tau_pl, mu_pl;  // iteration i - 1 (the head: n - 1)
     float alpha0, alpha1, eta_p, eta_q, lambda;
 };
@@ -346,8 +346,8 @@ __device__ __forceinline__ void fused_dual(const FusedArgs& a, int X, int Y, flo
     const bool fx = n.mE, fy = n.mS;
     const int ix = fx ? n.iE : n.iW, iy = fy ? n.iS : n.iN;
     dual_adj_pixel(fx, fy, a.ub1[idx], a.vb1[idx], a.ub1[ix], a.vb1[ix], a.ub1[iy], a.vb1[iy], a.abc[3 * idx],
-                   a.abc[3 * idx + 1], a.abc[3 * idx + 2], a.p1[idx], a.q1[idx], a.sigma, a.alpha0, a.alpha1, a.eta_p,
-                   a.eta_q, a.gp[idx], gqn, gtp, gpo, gqo, msg, gab, gpa0, gpa1);
+                   a.abc[3 * idx + 1], a.abc[3 * idx + 2], a.p1[idx], a.q1[idx], a.st, a.gp[idx], gqn, gtp, gpo, gqo,
+                   msg, gab, gpa0, gpa1);
 }
 
 template <int LX>
@@ -614,7 +614,8 @@ struct BlockRevArgs {
     const float2* vb1[KRMAX];
     const float* u2[KRMAX];
     const float* ub2[KRMAX];
-    float sigma[KRMAX], tau_pl[KRMAX], mu_pl[KRMAX];
+    DualSteps st[KRMAX];  // dual_steps of iteration i's sigma
+    float tau_pl[KRMAX], mu_pl[KRMAX];
     int n_units;
     int H, W, tiles_x, tiles_y;
     float alpha0, alpha1, eta_p, eta_q, lambda;
@@ -824,8 +825,8 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
                                make_float2(fsb[3 * E + ix], fsb[4 * E + ix]), fsb[2 * E + iy],
                                make_float2(fsb[3 * E + iy], fsb[4 * E + iy]), ca[j], cb[j], cc[j],
                                make_float2(p1x[j], p1y[j]),
-                               make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), a.sigma[u], a.alpha0, a.alpha1, a.eta_p,
-                               a.eta_q, make_float2(gpx[j], gpy[j]), gqn, gtp, gpo, gqo, m, ab[j], pa0, pa1);
+                               make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), a.st[u], make_float2(gpx[j], gpy[j]), gqn,
+                               gtp, gpo, gqo, m, ab[j], pa0, pa1);
                 if (f & RB_TILE) {
                     gpa0 += pa0;
                     gpa1 += pa1;
@@ -1347,7 +1348,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                 fa.ub2 = sim1.ub;
             };
             set_states(state_of(cs, n_iter), state_of(cs, n_iter - 1));
-            fa.sigma = 0.0f;
+            fa.st = dual_steps(0.0f, prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q);
             fa.tau_pl = taus[n_iter - 1];
             fa.mu_pl = mus[n_iter - 1];
             set_bufs();
@@ -1359,7 +1360,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                         VFS_TRY(recompute(cs));
                     }
                     set_states(state_of(cs, i), state_of(cs, i - 1));
-                    fa.sigma = sigmas[i];
+                    fa.st = dual_steps(sigmas[i], prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q);
                     fa.tau_pl = taus[i - 1];
                     fa.mu_pl = mus[i - 1];
                     set_bufs();
@@ -1416,7 +1417,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                         ba.vb1[u] = reinterpret_cast<const float2*>(si.vb);
                         ba.u2[u] = sm.u;
                         ba.ub2[u] = sm.ub;
-                        ba.sigma[u] = sigmas[iu];
+                        ba.st[u] = dual_steps(sigmas[iu], prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q);
                         ba.tau_pl[u] = taus[iu - 1];
                         ba.mu_pl[u] = mus[iu - 1];
                     }
@@ -1443,7 +1444,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
             }
             const State s0 = state_at(ws.ckpt0, n);
             set_states(s0, s0);
-            fa.sigma = sigmas[0];
+            fa.st = dual_steps(sigmas[0], prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q);
             set_bufs();
             VFS_TRY(launch_fused_rows(rows, 2, fblocks, s, fa));
         } else if (nck > 0) {
