@@ -120,7 +120,7 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter, levels=None):
     """Average duration and algorithmic bandwidth of the blocked primal-dual kernel over one solve's launches.
 
     Bytes per launch (compulsory traffic of the formulation): every pixel of the
-    tile extended by 2k (west, north) and k (east, south), clipped to the image,
+    tile extended by k on every side, clipped to the image,
     reads 73 B (u, u_bar, v_bar, p, q = 40 B; abc, eta, deriv = 32 B; mask 1 B;
     the warp's first launch reads 41 B, its v_bar, p, q being zero), and every
     tile pixel writes 40 B.  Each level's warp is recorded into a HIP graph and replayed
@@ -136,9 +136,9 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter, levels=None):
         def ext_px():
             tot = 0
             for y0 in range(0, hh, ty):
-                ey = min(hh, y0 + ty + k) - max(0, y0 - 2 * k)
+                ey = min(hh, y0 + ty + k) - max(0, y0 - k)
                 for x0 in range(0, ww, tx):
-                    tot += ey * (min(ww, x0 + tx + k) - max(0, x0 - 2 * k))
+                    tot += ey * (min(ww, x0 + tx + k) - max(0, x0 - k))
             return tot
 
         ext = ext_px()

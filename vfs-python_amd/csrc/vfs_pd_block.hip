@@ -2,15 +2,20 @@
 // fused per-warp kernels of the blocked level loop, gfx950.
 //
 // One iteration (dual -> threshold -> primal, Python/vfs.py:498-515) reads, at a
-// pixel, u_bar / v_bar at its four neighbours (dual), tp at its west and north
-// neighbours (threshold) and q at its west and north neighbours (primal).  The
-// region on which the state is exact therefore shrinks by one pixel on the
-// east/south side and by two on the west/north side per iteration.  A workgroup
-// that must produce a TX x TY tile after K iterations loads the tile extended by
-// 2K (west, north) and K (east, south), keeps each owned pixel's state and
-// constants in registers, exchanges the neighbour fields through LDS (two
-// barriers per iteration), and writes back only the tile.  Neighbouring tiles
-// read their halos from the same input buffers, so the state is ping-ponged
+// pixel, u_bar / v_bar at its east and south neighbours (the dual step's forward
+// differences; west / north where those are masked), and tp and q at its west and
+// north neighbours (threshold, primal), both taken from the dual step of the same
+// iteration.  The new u_bar of a pixel x therefore depends on the old u_bar at
+// x - 1 .. x + 1 only (through p(x - 1), which reads u_bar(x - 1) and u_bar(x), and
+// p(x), which reads u_bar(x + 1)), and p, q are exact wherever u_bar was: the
+// region on which the state is exact shrinks by one pixel on every side per
+// iteration.  (A missing neighbour at the region's border is replaced by the pixel
+// itself, as at the image border; only pixels within k of the border after k
+// iterations see the difference.)  A workgroup that must produce a TX x TY tile
+// after K iterations loads the tile extended by K on every side, keeps each owned
+// pixel's state and constants in registers, exchanges the neighbour fields through
+// LDS (two barriers per iteration), and writes back only the tile.  Neighbouring
+// tiles read their halos from the same input buffers, so the state is ping-ponged
 // between two buffer sets across launches.
 //
 // The per-pixel arithmetic is the same inline code as the per-kernel path
@@ -26,7 +31,7 @@ using namespace vfs;
 
 namespace {
 
-constexpr int KMAX = 8;
+constexpr int KMAX = 12;
 
 struct PDSched {
     float tau[KMAX];
@@ -88,14 +93,14 @@ __device__ __forceinline__ void pd_tape_write(const PDLevel& L, int k, int ox, i
                                               const float (&qx)[S], const float (&qy)[S], const float (&qz)[S],
                                               const float (&qw)[S])
 {
-    constexpr int EX = TX + 3 * K, EY = TY + 3 * K, E = EX * EY;
+    constexpr int EX = TX + 2 * K, EY = TY + 2 * K, E = EX * EY;
     float* base = L.tape + static_cast<int64_t>(k) * L.tape_stride;
 #pragma unroll
     for (int j = 0; j < S; ++j) {
         const int e = threadIdx.x + NT * j;
         if (e >= E) continue;
         const int ex = e % EX, ey = e / EX;
-        if (ex < 2 * K || ex >= 2 * K + TX || ey < 2 * K || ey >= 2 * K + TY) continue;
+        if (ex < K || ex >= K + TX || ey < K || ey >= K + TY) continue;
         if (!(fl[j] & F_IMG)) continue;
         const int64_t idx = static_cast<int64_t>(oy + ey) * L.W + (ox + ex);
         reinterpret_cast<float2*>(base)[idx] = make_float2(px[j], py[j]);
@@ -110,11 +115,11 @@ template <int TX, int TY, int K, int NT>
 __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, int k_count, const PDSched& sc,
                                               const PDConsts& cst, float* __restrict__ sA, float* __restrict__ sB)
 {
-    constexpr int EX = TX + 3 * K, EY = TY + 3 * K, E = EX * EY;
+    constexpr int EX = TX + 2 * K, EY = TY + 2 * K, E = EX * EY;
     constexpr int S = (E + NT - 1) / NT;
     const int tid = threadIdx.x;
     const int H = L.H, W = L.W;
-    const int ox = tx * TX - 2 * K, oy = ty * TY - 2 * K;
+    const int ox = tx * TX - K, oy = ty * TY - K;
 
     float u[S], ub[S], vbx[S], vby[S], px[S], py[S], qx[S], qy[S], qz[S], qw[S];
     // constants: abc, 1/eta (recip_eta), Iu, 1/Iu (recip_grad), Iz
@@ -290,7 +295,7 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
         const int e = tid + NT * j;
         if (e >= E) continue;
         const int ex = e % EX, ey = e / EX;
-        if (ex < 2 * K || ex >= 2 * K + TX || ey < 2 * K || ey >= 2 * K + TY) continue;
+        if (ex < K || ex >= K + TX || ey < K || ey >= K + TY) continue;
         if (!(fl[j] & F_IMG)) continue;
         const int idx = (oy + ey) * W + (ox + ex);
         L.u_out[idx] = u[j];
@@ -304,14 +309,14 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
 template <int TX, int TY, int K, int NT>
 constexpr int lds_floats()
 {
-    return 9 * (TX + 3 * K) * (TY + 3 * K);
+    return 9 * (TX + 2 * K) * (TY + 2 * K);
 }
 
 // Tiles of one level; blockIdx.x enumerates tiles row by row.
 template <int TX, int TY, int K, int NT>
 __global__ void __launch_bounds__(NT) k_pd_block(PDLevel L, int k_count, PDSched sc, PDConsts cst)
 {
-    constexpr int E = (TX + 3 * K) * (TY + 3 * K);
+    constexpr int E = (TX + 2 * K) * (TY + 2 * K);
     __shared__ float sA[3 * E];
     __shared__ float sB[6 * E];
     const int t = xcd_item(blockIdx.x, gridDim.x);
@@ -325,7 +330,7 @@ __global__ void __launch_bounds__(NT) k_pd_block_batched(const PDLevel* __restri
                                                          const int* __restrict__ tile_start, int n, int k_count,
                                                          PDSched sc, PDConsts cst)
 {
-    constexpr int E = (TX + 3 * K) * (TY + 3 * K);
+    constexpr int E = (TX + 2 * K) * (TY + 2 * K);
     __shared__ float sA[3 * E];
     __shared__ float sB[6 * E];
     const int t = xcd_item(blockIdx.x, gridDim.x);
@@ -509,29 +514,25 @@ struct Variant {
     {TX, TY, K, NT, k_pd_block<TX, TY, K, NT>, k_pd_block_batched<TX, TY, K, NT>}
 
 // Tile shapes and iteration counts available to the autotuner.  LDS use is
-// 36 bytes per pixel of the extended (TX + 3K) x (TY + 3K) region.
+// 36 bytes per pixel of the extended (TX + 2K) x (TY + 2K) region.
 const Variant kVariants[] = {
     VFS_PD_VARIANT(16, 16, 1, 256),  VFS_PD_VARIANT(16, 16, 2, 512),  VFS_PD_VARIANT(16, 16, 4, 1024),
-    VFS_PD_VARIANT(16, 16, 8, 1024), VFS_PD_VARIANT(32, 32, 1, 1024), VFS_PD_VARIANT(32, 32, 2, 1024),
-    VFS_PD_VARIANT(32, 32, 4, 1024),
-    VFS_PD_VARIANT(64, 16, 2, 1024), VFS_PD_VARIANT(64, 16, 4, 1024),
-    VFS_PD_VARIANT(32, 16, 4, 1024), VFS_PD_VARIANT(32, 16, 8, 1024), VFS_PD_VARIANT(8, 8, 8, 512),
-    VFS_PD_VARIANT(16, 16, 2, 256),  VFS_PD_VARIANT(32, 32, 2, 256),  VFS_PD_VARIANT(32, 32, 4, 256),
-    // small tiles: more workgroups for the coarse levels
-    VFS_PD_VARIANT(8, 8, 1, 128),    VFS_PD_VARIANT(8, 8, 2, 256),    VFS_PD_VARIANT(8, 8, 4, 512),
-    VFS_PD_VARIANT(16, 8, 2, 256),   VFS_PD_VARIANT(16, 8, 4, 512),   VFS_PD_VARIANT(16, 16, 4, 512),
-    VFS_PD_VARIANT(32, 8, 2, 256),   VFS_PD_VARIANT(32, 8, 4, 512),   VFS_PD_VARIANT(32, 16, 2, 512),
-    // tiles whose extended region fills the workgroup exactly (no idle lanes): 26 + 6 = 32,
-    // 29 + 3 = 32, 20 + 12 = 32, 58 + 6 = 64
-    VFS_PD_VARIANT(26, 26, 2, 1024), VFS_PD_VARIANT(29, 29, 1, 1024), VFS_PD_VARIANT(20, 20, 4, 1024),
-    VFS_PD_VARIANT(58, 26, 2, 1024), VFS_PD_VARIANT(26, 10, 2, 512),  VFS_PD_VARIANT(13, 13, 1, 256),
-    // exact fits for three iterations (23 + 9 = 32, 55 + 9 = 64) and a region of two pixels
-    // per lane for four (52 + 12 = 64 by 20 + 12 = 32); four pixels per lane (52 x 52) spills
-    VFS_PD_VARIANT(23, 23, 3, 1024), VFS_PD_VARIANT(55, 23, 3, 1024), VFS_PD_VARIANT(23, 7, 3, 512),
-    VFS_PD_VARIANT(52, 20, 4, 1024),
-    // deeper exact fits for the latency-bound coarse levels: fewer launches per warp
-    // (17 + 15 = 14 + 18 = 8 + 24 = 32)
-    VFS_PD_VARIANT(17, 17, 5, 1024), VFS_PD_VARIANT(14, 14, 6, 1024), VFS_PD_VARIANT(8, 8, 8, 1024),
+    VFS_PD_VARIANT(32, 32, 1, 1024), VFS_PD_VARIANT(32, 32, 2, 1024), VFS_PD_VARIANT(32, 32, 4, 1024),
+    VFS_PD_VARIANT(64, 16, 2, 1024), VFS_PD_VARIANT(64, 16, 4, 1024), VFS_PD_VARIANT(32, 16, 4, 1024),
+    VFS_PD_VARIANT(32, 16, 8, 1024), VFS_PD_VARIANT(8, 8, 8, 512),    VFS_PD_VARIANT(16, 16, 2, 256),
+    VFS_PD_VARIANT(8, 8, 1, 128),    VFS_PD_VARIANT(8, 8, 2, 256),    VFS_PD_VARIANT(16, 8, 2, 256),
+    // tiles whose extended region fills the workgroup exactly (no idle lanes),
+    // 1024 lanes on 32 x 32: TX + 2K = 32
+    VFS_PD_VARIANT(30, 30, 1, 1024), VFS_PD_VARIANT(28, 28, 2, 1024), VFS_PD_VARIANT(26, 26, 3, 1024),
+    VFS_PD_VARIANT(24, 24, 4, 1024), VFS_PD_VARIANT(22, 22, 5, 1024), VFS_PD_VARIANT(20, 20, 6, 1024),
+    VFS_PD_VARIANT(18, 18, 7, 1024), VFS_PD_VARIANT(16, 16, 8, 1024),
+    // deeper, for the latency-bound coarse levels: fewer launches per warp
+    VFS_PD_VARIANT(12, 12, 10, 1024), VFS_PD_VARIANT(8, 8, 12, 1024),
+    // two pixels per lane (64 x 32), 512 lanes on 32 x 16, 256 lanes on 16 x 16
+    VFS_PD_VARIANT(60, 28, 2, 1024), VFS_PD_VARIANT(58, 26, 3, 1024), VFS_PD_VARIANT(56, 24, 4, 1024),
+    VFS_PD_VARIANT(28, 12, 2, 512),  VFS_PD_VARIANT(26, 10, 3, 512),  VFS_PD_VARIANT(24, 8, 4, 512),
+    VFS_PD_VARIANT(14, 14, 1, 256),  VFS_PD_VARIANT(12, 12, 2, 256),  VFS_PD_VARIANT(10, 10, 3, 256),
+    VFS_PD_VARIANT(8, 8, 4, 256),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
