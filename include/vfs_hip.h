@@ -416,9 +416,9 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
  *     messages of a 32 x 8 tile and its one-pixel ring exchanged in LDS); 1: one
  *     kernel per step (vfs_adj_pd_iteration), messages through memory; 11, 12, 14:
  *     fused with tiles of 32 x 8, 32 x 16, 32 x 32 (1, 2, 4 rows per thread);
- *     21-24: temporally blocked, K = 1-4 consecutive fused units per launch on a
+ *     21-26: temporally blocked, K = 1-6 consecutive fused units per launch on a
  *     32 x 32 extended tile (1024 threads), the adjoint state in registers and the
- *     messages in LDS; 31, 32: the same with K = 1, 2 on 32 x 16 (512 threads).
+ *     messages in LDS; 31-33: the same with K = 1-3 on 32 x 16 (512 threads).
  *     The blocked modes keep the fused kernel for the first and last unit of a warp;
  *   reverse_warps: 0 = the exact reverse of every warp; r > 0 = only the last r warps
  *     of the level are reversed and their start state is held fixed (the adjoints of

@@ -565,13 +565,19 @@ int launch_fused_rows(int rows, int kind, int nblocks, hipStream_t s, const Fuse
 
 // ---------------------------------------------------------------------------
 // Temporally blocked reverse sweep: up to KR consecutive units F(i) = dual^T(i) +
-// primal^T / threshold^T(i - 1) per launch, on a tile extended by KR pixels west /
-// north and 2 KR east / south.
+// primal^T / threshold^T(i - 1) per launch, on a tile extended by KR pixels on every
+// side.
 //
 // Per unit the region on which the adjoint state is exact shrinks by one pixel on
-// the east / south side (dual^T reads the E and S messages of the previous unit) and
-// by one more on every side (primal^T gathers the dual^T messages of x +- 1, y +- 1),
-// so after KR units the tile itself is exact.  The adjoint state (gu, g u_bar,
+// every side.  dual^T at x reads the messages of x and x + 1 (E; S likewise), and
+// primal^T at x gathers the dual^T messages of x - 1 (whose dual step read x as its
+// east neighbour) and of x + 1 only when x + 1 read x as its west neighbour, i.e.
+// when x + 2 is outside the mask; the messages of such a pixel are zero, so that
+// sender is exact wherever x + 1 is.  A neighbour beyond the loaded region
+// contributes nothing (or is the pixel itself, for the forward state read): the
+// pixels within one of the region's border are inexact after the first unit, and
+// the inexact ring grows by one per unit, so after KR units the tile itself is
+// exact.  The adjoint state (gu, g u_bar,
 // g v_bar, gp, gq) and the accumulators of abc, eta and the derivatives stay in
 // registers; the messages and, per unit, the neighbours' forward state go through LDS
 // (each pixel loads its own state from the tape).  Per-pixel arithmetic is the
@@ -580,7 +586,7 @@ int launch_fused_rows(int rows, int kind, int nblocks, hipStream_t s, const Fuse
 // grouped differently.
 // ---------------------------------------------------------------------------
 
-constexpr int KRMAX = 4;
+constexpr int KRMAX = 6;
 
 struct BlockRevArgs {
     // adjoint state read (one set) and written (the other): every pixel of the level
@@ -638,7 +644,7 @@ enum : unsigned {
 template <int TX, int TY, int KR, int NT>
 __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
 {
-    constexpr int EX = TX + 3 * KR, EY = TY + 3 * KR, E = EX * EY, S = (E + NT - 1) / NT;
+    constexpr int EX = TX + 2 * KR, EY = TY + 2 * KR, E = EX * EY, S = (E + NT - 1) / NT;
     __shared__ float sQM[2 * E];     // primal^T messages (-g_S1, -g_S2), compact
     __shared__ float sTM[E];         // threshold^T messages g_g
     __shared__ float sDM[9 * E];     // dual^T messages (DualMsg um[3], vm[6])
@@ -987,14 +993,15 @@ __global__ void __launch_bounds__(256) k_fold_gpart_tiles(float* __restrict__ gp
     g[GP_ALPHA1] += static_cast<float>(gpart[3 * b + 2]);
 }
 
-// Reverse variants (sweep modes): 20 + KR with 1024 threads on a 32 x 32 extended
-// tile, 30 + KR (one or two units) with 512 threads on 32 x 16.
+// Reverse variants (sweep modes): 20 + KR (1-6 units) with 1024 threads on a 32 x 32
+// extended tile, 30 + KR (1-3 units) with 512 threads on 32 x 16.
 struct RevVariant {
     int mode, tx, ty, kr, nt;
 };
 constexpr RevVariant kRevVariants[] = {
-    {21, 29, 29, 1, 1024}, {22, 26, 26, 2, 1024}, {23, 23, 23, 3, 1024}, {24, 20, 20, 4, 1024},
-    {31, 29, 13, 1, 512},  {32, 26, 10, 2, 512},
+    {21, 30, 30, 1, 1024}, {22, 28, 28, 2, 1024}, {23, 26, 26, 3, 1024}, {24, 24, 24, 4, 1024},
+    {25, 22, 22, 5, 1024}, {26, 20, 20, 6, 1024}, {31, 30, 14, 1, 512},  {32, 28, 12, 2, 512},
+    {33, 26, 10, 3, 512},
 };
 
 const RevVariant* find_rev_variant(int mode)
@@ -1014,12 +1021,15 @@ int launch_adj_block_t(hipStream_t s, int tiles, const BlockRevArgs& ba)
 int launch_adj_block(const RevVariant& v, hipStream_t s, int tiles, const BlockRevArgs& ba)
 {
     switch (v.mode) {
-        case 21: return launch_adj_block_t<29, 29, 1, 1024>(s, tiles, ba);
-        case 22: return launch_adj_block_t<26, 26, 2, 1024>(s, tiles, ba);
-        case 23: return launch_adj_block_t<23, 23, 3, 1024>(s, tiles, ba);
-        case 24: return launch_adj_block_t<20, 20, 4, 1024>(s, tiles, ba);
-        case 31: return launch_adj_block_t<29, 13, 1, 512>(s, tiles, ba);
-        case 32: return launch_adj_block_t<26, 10, 2, 512>(s, tiles, ba);
+        case 21: return launch_adj_block_t<30, 30, 1, 1024>(s, tiles, ba);
+        case 22: return launch_adj_block_t<28, 28, 2, 1024>(s, tiles, ba);
+        case 23: return launch_adj_block_t<26, 26, 3, 1024>(s, tiles, ba);
+        case 24: return launch_adj_block_t<24, 24, 4, 1024>(s, tiles, ba);
+        case 25: return launch_adj_block_t<22, 22, 5, 1024>(s, tiles, ba);
+        case 26: return launch_adj_block_t<20, 20, 6, 1024>(s, tiles, ba);
+        case 31: return launch_adj_block_t<30, 14, 1, 512>(s, tiles, ba);
+        case 32: return launch_adj_block_t<28, 12, 2, 512>(s, tiles, ba);
+        case 33: return launch_adj_block_t<26, 10, 3, 512>(s, tiles, ba);
         default: return VFS_EINVAL;
     }
 }

@@ -58,11 +58,11 @@ class VFSDepthAdjoint:
         # 10 + rows = fused with that many rows per thread
         self._mode = 1 if not self.fused else (0 if rows is None else 10 + rows)
         self._auto_block = False
-        # temporally blocked sweep: reverse_block = K units per launch on a 32 x 32 extended
-        # tile (1024 threads; sweep 20 + K), or (K, 512) for one or two units on 32 x 16;
+        # temporally blocked sweep: reverse_block = K (1-6) units per launch on a 32 x 32
+        # extended tile (1024 threads; sweep 20 + K), or (K, 512) for 1-3 units on 32 x 16;
         # a dict {level: choice} sets it per pyramid level (missing levels: fused); "auto"
-        # picks by level size (three units at >= 400 k pixels, two below, and two on the
-        # 512-thread tile under 100 k: measured at the 800 x 800 gradient workload)
+        # picks by level size (three units at >= 100 k pixels, five below: measured at the
+        # 800 x 800 gradient workload)
         self._level_mode = {}
         if reverse_block == "auto":
             reverse_block = None if (not self.fused or rows is not None) else "auto"
@@ -71,7 +71,7 @@ class VFSDepthAdjoint:
         elif reverse_block is not None:
             if not self.fused or rows is not None:
                 raise ValueError("reverse_block replaces the fused sweep's middle launches; keep fused=True, rows=None")
-            modes = {1: 21, 2: 22, 3: 23, 4: 24, (1, 512): 31, (2, 512): 32}
+            modes = {1: 21, 2: 22, 3: 23, 4: 24, 5: 25, 6: 26, (1, 512): 31, (2, 512): 32, (3, 512): 33}
             per = reverse_block if isinstance(reverse_block, dict) else None
             for lvl, rb in (per.items() if per is not None else [(None, reverse_block)]):
                 key = rb if isinstance(rb, int) else tuple(rb)
@@ -135,7 +135,7 @@ class VFSDepthAdjoint:
             t.rec_tile_x, t.rec_tile_y, t.rec_k, t.rec_threads = rec
         if self._auto_block:
             npx = hh * ww
-            t.sweep = 23 if npx >= 400_000 else (22 if npx >= 100_000 else 32)
+            t.sweep = 23 if npx >= 100_000 else 25
         else:
             t.sweep = self._level_mode.get(k, self._mode)
         t.reverse_warps = self.reverse_warps
