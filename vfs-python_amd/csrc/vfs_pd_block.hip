@@ -206,10 +206,12 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
         if (sB[(f & F_HN) ? e - EX : e] != 0.0f) g |= F_MN;
         fl[j] = g;
     }
-    if (skip) {
+    // Outside the mask every state variable is zero after the first iteration.  u, p and
+    // q were never loaded there and the steps below write them only inside the mask; u_bar
+    // and v_bar, which the first dual step reads through sA as loaded, are zeroed now.
 #pragma unroll
-        for (int j = 0; j < S; ++j) ub[j] = vbx[j] = vby[j] = 0.0f;
-    }
+    for (int j = 0; j < S; ++j)
+        if (skip || !(fl[j] & F_ACT)) ub[j] = vbx[j] = vby[j] = 0.0f;
 
     for (int k = 0; k < (skip ? 0 : k_count); ++k) {
         const float tau = sc.tau[k], mu = sc.mu[k], sp = sc.sp[k], sq = sc.sq[k];
@@ -220,7 +222,6 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
             const int e = tid + NT * j;
             if (e >= E) continue;
             const unsigned f = fl[j];
-            float tpx = 0.0f, tpy = 0.0f;
             if (f & F_ACT) {
                 const bool fx = (f & F_ME) != 0, fy = (f & F_MS) != 0;
                 const int eE = (f & F_HE) ? e + 1 : e, eW = (f & F_HW) ? e - 1 : e;
@@ -234,17 +235,14 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
                            tpn, qn);
                 px[j] = pn.x;
                 py[j] = pn.y;
-                tpx = tpn.x;
-                tpy = tpn.y;
                 qx[j] = qn.x;
                 qy[j] = qn.y;
                 qz[j] = qn.z;
                 qw[j] = qn.w;
-            } else {
-                px[j] = py[j] = qx[j] = qy[j] = qz[j] = qw[j] = 0.0f;
             }
-            sB[e] = tpx;
-            sB[E + e] = tpy;
+            // T p (dual_pixel's tpn; zero outside the mask, where p and abc are zero)
+            sB[e] = fmaf(ca[j], px[j], cc[j] * py[j]);
+            sB[E + e] = fmaf(cc[j], px[j], cb[j] * py[j]);
             sB[2 * E + e] = qx[j];
             sB[3 * E + e] = qy[j];
             sB[4 * E + e] = qz[j];
@@ -262,9 +260,10 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
                 const bool mE = (f & F_ME) != 0, mW = (f & F_MW) != 0, mS = (f & F_MS) != 0, mN = (f & F_MN) != 0;
                 u[j] = l1_pixel(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j], riu[j],
                                 iz[j], r0[j], u[j], ub[j], tau, cst.lambda);
-                const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                const float4 qW = mW ? make_float4(sB[2 * E + eW], sB[3 * E + eW], sB[4 * E + eW], sB[5 * E + eW]) : zero;
-                const float4 qN = mN ? make_float4(sB[2 * E + eN], sB[3 * E + eN], sB[4 * E + eN], sB[5 * E + eN]) : zero;
+                // read whether or not the neighbour is inside the mask: primal_pixel uses
+                // qW (qN) only when it is
+                const float4 qW = make_float4(sB[2 * E + eW], sB[3 * E + eW], sB[4 * E + eW], sB[5 * E + eW]);
+                const float4 qN = make_float4(sB[2 * E + eN], sB[3 * E + eN], sB[4 * E + eN], sB[5 * E + eN]);
                 float ubn;
                 float2 vbn;
                 primal_pixel(make_float4(qx[j], qy[j], qz[j], qw[j]), qW, qN, mE, mW, mS, mN, ca[j], cb[j], cc[j],
@@ -273,8 +272,6 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
                 ub[j] = ubn;
                 vbx[j] = vbn.x;
                 vby[j] = vbn.y;
-            } else {
-                u[j] = ub[j] = vbx[j] = vby[j] = 0.0f;
             }
             sA[e] = ub[j];
             sA[E + e] = vbx[j];
