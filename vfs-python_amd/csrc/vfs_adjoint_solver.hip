@@ -641,6 +641,37 @@ enum : unsigned {
     RB_TILE = 1024u,
 };
 
+// A pixel's forward state read by unit u from the tape: S_i (u_bar, v_bar, and inside
+// the mask p, q) and, inside the mask, u and u_bar of S_{i-1}.  Zero where not read.
+struct UnitTape {
+    float ub1, vb1x, vb1y, p1x, p1y, q1x, q1y, q1z, q1w, u2, ub2;
+};
+
+__device__ __forceinline__ UnitTape load_unit_tape(const BlockRevArgs& a, int u, unsigned f, int idx)
+{
+    UnitTape t;
+    t.ub1 = t.vb1x = t.vb1y = t.p1x = t.p1y = t.q1x = t.q1y = t.q1z = t.q1w = t.u2 = t.ub2 = 0.0f;
+    if (f & RB_IMG) {
+        t.ub1 = a.ub1[u][idx];
+        const float2 vb = a.vb1[u][idx];
+        t.vb1x = vb.x;
+        t.vb1y = vb.y;
+        if (f & RB_M) {
+            const float2 p = a.p1[u][idx];
+            const float4 q = a.q1[u][idx];
+            t.p1x = p.x;
+            t.p1y = p.y;
+            t.q1x = q.x;
+            t.q1y = q.y;
+            t.q1z = q.z;
+            t.q1w = q.w;
+            t.u2 = a.u2[u][idx];
+            t.ub2 = a.ub2[u][idx];
+        }
+    }
+    return t;
+}
+
 template <int TX, int TY, int KR, int NT>
 __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
 {
@@ -739,33 +770,41 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
         }
     }
 
+    // The tape of unit u + 1 is loaded while unit u runs (the loads stay in flight
+    // across the unit's barriers), so its latency is not on the unit's path.
+    UnitTape nxt[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) nxt[j] = load_unit_tape(a, 0, fl[j], gi[j]);
     for (int u = 0; u < a.n_units; ++u) {
         float* fsb = sFS[u & 1];
         // own forward state of this unit
-        float p1x[S], p1y[S], q1x[S], q1y[S], q1z[S], q1w[S], ub1[S], vb1x[S], vb1y[S];
+        float p1x[S], p1y[S], q1x[S], q1y[S], q1z[S], q1w[S], ub1[S], vb1x[S], vb1y[S], u2[S], ub2[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const UnitTape t = nxt[j];
+            p1x[j] = t.p1x;
+            p1y[j] = t.p1y;
+            q1x[j] = t.q1x;
+            q1y[j] = t.q1y;
+            q1z[j] = t.q1z;
+            q1w[j] = t.q1w;
+            ub1[j] = t.ub1;
+            vb1x[j] = t.vb1x;
+            vb1y[j] = t.vb1y;
+            u2[j] = t.u2;
+            ub2[j] = t.ub2;
+        }
+        if (u + 1 < a.n_units) {
+#pragma unroll
+            for (int j = 0; j < S; ++j) nxt[j] = load_unit_tape(a, u + 1, fl[j], gi[j]);
+        }
 #pragma unroll
         for (int j = 0; j < S; ++j) {
             const int e = tid + NT * j;
-            p1x[j] = p1y[j] = q1x[j] = q1y[j] = q1z[j] = q1w[j] = ub1[j] = vb1x[j] = vb1y[j] = 0.0f;
             float tx_ = 0.0f, ty_ = 0.0f;
-            if (fl[j] & RB_IMG) {
-                const int idx = gi[j];
-                ub1[j] = a.ub1[u][idx];
-                const float2 vb = a.vb1[u][idx];
-                vb1x[j] = vb.x;
-                vb1y[j] = vb.y;
-                if (fl[j] & RB_M) {
-                    const float2 p = a.p1[u][idx];
-                    const float4 q = a.q1[u][idx];
-                    p1x[j] = p.x;
-                    p1y[j] = p.y;
-                    q1x[j] = q.x;
-                    q1y[j] = q.y;
-                    q1z[j] = q.z;
-                    q1w[j] = q.w;
-                    tx_ = fmaf(ca[j], p.x, cc[j] * p.y);  // tp_at
-                    ty_ = fmaf(cc[j], p.x, cb[j] * p.y);
-                }
+            if (fl[j] & RB_M) {
+                tx_ = fmaf(ca[j], p1x[j], cc[j] * p1y[j]);  // tp_at
+                ty_ = fmaf(cc[j], p1x[j], cb[j] * p1y[j]);
             }
             if (e < E) {
                 fsb[e] = tx_;
@@ -914,7 +953,7 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
                              e2[j], a.tau_pl[u], a.mu_pl[u], a.alpha0, a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_,
                              qm, ab[j], et[j][1], et[j][2], pa0, pa1, qmc);
             gu_ = l1_adj_pixel(n, gu_, make_float2(fsb[e], fsb[E + e]), fsb[eW], fsb[E + eN], make_float2(iu[j], iz[j]),
-                               e0[j], a.u2[u][gi[j]], a.ub2[u][gi[j]], a.tau_pl[u], a.lambda, tm, ubB, et[j][0], gd,
+                               e0[j], u2[j], ub2[j], a.tau_pl[u], a.lambda, tm, ubB, et[j][0], gd,
                                pl, g_g);
             dd[j][0] = gd.x;
             dd[j][1] = gd.y;
