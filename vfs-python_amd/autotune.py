@@ -22,16 +22,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "autotune_gfx950.json")
 
 _table = None
+_batch_table = None
 
 
 def _load():
-    global _table
+    global _table, _batch_table
     if _table is None:
-        _table = {}
+        _table, _batch_table = {}, {}
         if os.path.exists(TABLE):
             with open(TABLE) as f:
-                for row in json.load(f).get("entries", []):
-                    _table[(int(row["h"]), int(row["w"]), int(row["iters"]))] = variant(row["best"])
+                doc = json.load(f)
+            for row in doc.get("entries", []):
+                _table[(int(row["h"]), int(row["w"]), int(row["iters"]))] = variant(row["best"])
+            for row in doc.get("batch_entries", []):
+                _batch_table[(int(row["pixels"]), int(row["iters"]))] = variant(row["best"])
     return _table
 
 
@@ -68,9 +72,15 @@ def tile_for(h, w, n_iter):
 
 
 def tile_for_pixels(n, n_iter):
-    """The tuned variant of the table entry whose pixel count is nearest to n (log scale):
-    the choice for one launch over n pixels spread across many pairs."""
+    """The variant for one launch over n pixels spread across many pairs: the batch
+    entry (measured on whole batched solves) whose pixel count is nearest to n on a log
+    scale, for n within the batch entries' range; else the single-level entry nearest
+    to n."""
     t = _load()
+    if _batch_table and n >= min(k[0] for k in _batch_table) // 2:
+        best = min(_batch_table.items(), key=lambda kv: (abs(math.log(kv[0][0]) - math.log(max(n, 1))),
+                                                         abs(kv[0][1] - n_iter)))
+        return best[1]
     if not t:
         side = int(math.sqrt(n))
         return default_tile(side, side)
@@ -182,15 +192,16 @@ def tune(sizes, n_iter, reps=5, rounds=3, log=print):
 
 
 def save(entries, path=TABLE):
-    old = []
+    doc = {"arch": "gfx950"}
     if os.path.exists(path):
         with open(path) as f:
-            old = json.load(f).get("entries", [])
-    keep = {(e["h"], e["w"], e["iters"]): e for e in old}
+            doc = json.load(f)
+    keep = {(e["h"], e["w"], e["iters"]): e for e in doc.get("entries", [])}
     for e in entries:
         keep[(e["h"], e["w"], e["iters"])] = e
+    doc["entries"] = sorted(keep.values(), key=lambda e: -e["h"] * e["w"])
     with open(path, "w") as f:
-        json.dump({"arch": "gfx950", "entries": sorted(keep.values(), key=lambda e: -e["h"] * e["w"])}, f, indent=1)
+        json.dump(doc, f, indent=1)
     global _table
     _table = None
 

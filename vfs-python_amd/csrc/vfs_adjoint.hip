@@ -15,6 +15,7 @@
 // min(d, limit) passes the gradient to d when d < limit; the threshold uses
 // the branch its forward took; the median routes to the selected element.
 #include "vfs_adj_pixel.h"
+#include "vfs_median.h"
 
 using namespace vfs;
 
@@ -225,7 +226,6 @@ __global__ void __launch_bounds__(256) k_adj_clamp(float* __restrict__ gu, float
         return;
     }
     constexpr int N = KM * KM;
-    constexpr int M = N / 2;
     float v[N], w[N];
 #pragma unroll
     for (int j = 0; j < KM; ++j) {
@@ -233,17 +233,7 @@ __global__ void __launch_bounds__(256) k_adj_clamp(float* __restrict__ gu, float
 #pragma unroll
         for (int i = 0; i < KM; ++i) w[j * KM + i] = v[j * KM + i] = u_th[yy * W + clamp_idx(X + i - KM / 2, W)];
     }
-#pragma unroll
-    for (int j = 0; j <= M; ++j) {
-#pragma unroll
-        for (int i = j + 1; i < N; ++i) {
-            const float lo = fminf(v[j], v[i]);
-            const float hi = fmaxf(v[j], v[i]);
-            v[j] = lo;
-            v[i] = hi;
-        }
-    }
-    const float med = v[M];
+    const float med = median_select<KM>(v);
     int k = 0;
 #pragma unroll
     for (int i = N - 1; i >= 0; --i)

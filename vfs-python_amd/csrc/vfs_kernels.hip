@@ -5,6 +5,7 @@
 // implements.
 #include "vfs_device.h"
 #include "vfs_pd.h"
+#include "vfs_median.h"
 
 using namespace vfs;
 
@@ -268,7 +269,6 @@ __global__ void __launch_bounds__(256) k_median(float* __restrict__ out, const f
         return;
     }
     constexpr int N = K * K;
-    constexpr int M = N / 2;
     float v[N];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -279,17 +279,7 @@ __global__ void __launch_bounds__(256) k_median(float* __restrict__ out, const f
             v[j * K + i] = in[yy * W + xx];
         }
     }
-#pragma unroll
-    for (int j = 0; j <= M; ++j) {
-#pragma unroll
-        for (int i = j + 1; i < N; ++i) {
-            const float lo = fminf(v[j], v[i]);
-            const float hi = fmaxf(v[j], v[i]);
-            v[j] = lo;
-            v[i] = hi;
-        }
-    }
-    out[idx] = v[M];
+    out[idx] = median_select<K>(v);
 }
 
 // computeOpticalFlowWithClamp, Python/vfs.cu:641-676 (in place on flow and u0).

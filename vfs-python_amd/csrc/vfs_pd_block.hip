@@ -26,6 +26,7 @@
 #include "vfs_device.h"
 #include "vfs_internal.h"
 #include "vfs_pd.h"
+#include "vfs_median.h"
 
 using namespace vfs;
 
@@ -385,7 +386,6 @@ __device__ __forceinline__ void median_clamp_prep_pixel(int X, int Y, float* __r
     float2 f = flow[idx];
     if (mask[idx] != 0) {
         constexpr int N = KM * KM;
-        constexpr int M = N / 2;
         float v[N];
 #pragma unroll
         for (int j = 0; j < KM; ++j) {
@@ -393,18 +393,9 @@ __device__ __forceinline__ void median_clamp_prep_pixel(int X, int Y, float* __r
 #pragma unroll
             for (int i = 0; i < KM; ++i) v[j * KM + i] = u_th[yy * W + clamp_idx(X + i - KM / 2, W)];
         }
-#pragma unroll
-        for (int j = 0; j <= M; ++j) {
-#pragma unroll
-            for (int i = j + 1; i < N; ++i) {
-                const float lo = fminf(v[j], v[i]);
-                const float hi = fmaxf(v[j], v[i]);
-                v[j] = lo;
-                v[i] = hi;
-            }
-        }
+        const float med = median_select<KM>(v);
         const float base = u_ws[idx];
-        const float delta = fminf(v[M] - base, limit);
+        const float delta = fminf(med - base, limit);
         u_ws[idx] = base + delta;
         const float2 t = tv[idx];
         f.x = fmaf(delta, t.x, f.x);
