@@ -1,5 +1,5 @@
 // Median of a 3 x 3 or 5 x 5 window: the selection shared by every median filter of
-// the solver (medianFilter, Python/vfs.cu:602-639) and its adjoint, so that all of them
+// the solver (medianFilter, Python/vfs.cu:584-639) and its adjoint, so that all of them
 // select the same element.
 //
 // 5 x 5: a selection network of 113 comparators (101 min and 101 max operations).  It is
