@@ -770,6 +770,14 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
         }
     }
 
+    // Outside the mask every unit leaves the adjoint state zero (dual^T and primal^T
+    // produce nothing there), so those lanes' registers are zeroed once here and not
+    // touched again; only their messages in LDS, which neighbours read as loaded in the
+    // first unit, are cleared by primal^T below.
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+        if (!(fl[j] & RB_M)) gu[j] = gub[j] = gvx[j] = gvy[j] = gpx[j] = gpy[j] = gqx[j] = gqy[j] = gqz[j] = gqw[j] = 0.0f;
+
     // The tape of unit u + 1 is loaded while unit u runs (the loads stay in flight
     // across the unit's barriers), so its latency is not on the unit's path.
     UnitTape nxt[S];
@@ -825,9 +833,10 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             const int e = tid + NT * j;
             if (e >= E) continue;
             const unsigned f = fl[j];
+            // outside the mask: gp, gq stay zero, and no pixel reads this lane's dual
+            // messages (primal^T gathers only from senders inside the mask)
+            if (!(f & RB_M)) continue;
             DualMsg m;
-            for (int c = 0; c < 3; ++c) m.um[c] = 0.0f;
-            for (int c = 0; c < 6; ++c) m.vm[c] = 0.0f;
             float2 gpo = make_float2(0.0f, 0.0f);
             float4 gqo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (f & RB_M) {
@@ -894,7 +903,6 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             if (e >= E) continue;
             const unsigned f = fl[j];
             if (!(f & RB_M)) {
-                gub[j] = gvx[j] = gvy[j] = gu[j] = 0.0f;
                 sQM[e] = sQM[E + e] = 0.0f;
                 sTM[e] = 0.0f;
                 continue;
@@ -933,7 +941,6 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             n.mW = (f & RB_MW) != 0;
             n.mS = (f & RB_MS) != 0;
             n.mN = (f & RB_MN) != 0;
-            const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             // W / N neighbours (the pixel itself at the image border, or beyond the region)
             const int eW = (gx > 0 && ex > 0) ? e - 1 : e, eN = (gy > 0 && ey > 0) ? e - EX : e;
             float pa0 = 0.0f, pa1 = 0.0f, pl = 0.0f;
@@ -944,10 +951,10 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             float2 gp_ = make_float2(gpx[j], gpy[j]);
             float4 gq_ = make_float4(gqx[j], gqy[j], gqz[j], gqw[j]);
             float gu_ = gu[j];
-            float4 qW = make_float4(fsb[5 * E + eW], fsb[6 * E + eW], fsb[7 * E + eW], fsb[8 * E + eW]);
-            float4 qN = make_float4(fsb[5 * E + eN], fsb[6 * E + eN], fsb[7 * E + eN], fsb[8 * E + eN]);
-            if (!n.mW) qW = zero4;
-            if (!n.mN) qN = zero4;
+            // read whether or not the neighbour is inside the mask: primal_adj_pixel uses
+            // qW (qN) only when it is
+            const float4 qW = make_float4(fsb[5 * E + eW], fsb[6 * E + eW], fsb[7 * E + eW], fsb[8 * E + eW]);
+            const float4 qN = make_float4(fsb[5 * E + eN], fsb[6 * E + eN], fsb[7 * E + eN], fsb[8 * E + eN]);
             primal_adj_pixel(n, make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), qW, qN, ca[j], cb[j], cc[j],
                              make_float2(p1x[j], p1y[j]), e1[j],
                              e2[j], a.tau_pl[u], a.mu_pl[u], a.alpha0, a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_,
