@@ -61,8 +61,9 @@ class VFSDepthAdjoint:
         # temporally blocked sweep: reverse_block = K (1-6) units per launch on a 32 x 32
         # extended tile (1024 threads; sweep 20 + K), or (K, 512) for 1-3 units on 32 x 16;
         # a dict {level: choice} sets it per pyramid level (missing levels: fused); "auto"
-        # picks by level size (three units at >= 100 k pixels, five below: measured at the
-        # 800 x 800 gradient workload)
+        # picks by level size (three units on the 512-thread tile at >= 400 k pixels, three
+        # on the 1024-thread tile at >= 100 k, six below: measured at the 800 x 800 gradient
+        # workload)
         self._level_mode = {}
         if reverse_block == "auto":
             reverse_block = None if (not self.fused or rows is not None) else "auto"
@@ -135,7 +136,7 @@ class VFSDepthAdjoint:
             t.rec_tile_x, t.rec_tile_y, t.rec_k, t.rec_threads = rec
         if self._auto_block:
             npx = hh * ww
-            t.sweep = 23 if npx >= 100_000 else 25
+            t.sweep = 33 if npx >= 400_000 else (23 if npx >= 100_000 else 26)
         else:
             t.sweep = self._level_mode.get(k, self._mode)
         t.reverse_warps = self.reverse_warps
