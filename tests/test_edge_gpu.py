@@ -16,7 +16,8 @@ Checks:
     2.9x the restatement's;
   * flow is exactly zero outside the mask (Python/vfs.py:531);
   * a fully masked pair gives zero flow in every mode, and triangulation gives the
-    masked sentinel (0, 0, -1) at every pixel (Python/vfs.cu:722-725).
+    masked sentinel (0, 0, -1) at every pixel (Python/vfs.cu:722-725);
+  * tiny pairs solved in one batch with a larger one equal their single-pair solves.
 """
 
 import numpy as np
@@ -102,3 +103,37 @@ def test_fully_masked_pair_gives_zero_flow_and_sentinel_depth(T):
     cam.setup_module()
     xyz = v.triangulate_flow(cam, np.array([0.2, 0.0, 0.0], np.float32)).cpu().numpy()
     assert np.all(xyz == np.array([0, 0, -1], np.float32))
+
+
+def test_batch_with_tiny_pairs_equals_single_solves(T):
+    """Pairs of 9x7, 5x13 and 64x48 (levels down to 2x6 and 4x3) in one batch: each
+    pair's flow equals its own blocked solve bit for bit, replays included."""
+    import copy
+    import vfs as V
+    import vfs_batch as B
+    import vfs_params as P
+    sizes = [(9, 7), (5, 13), (64, 48)]
+    cases = [_case(h, w, 2, 10 + i) for i, (h, w) in enumerate(sizes)]
+    prm = _params(P.vfs_params(), cases[0])
+    tiles = {0: (16, 16, 4), 1: (12, 12, 10)}
+    batch = B.VFSBatch(prm, sizes)
+    batch.pd_tiles = tiles
+    for i, c in enumerate(cases):
+        batch.setup_pair(i, c["mask"], c["tvf"], c["cvf"])
+    refs, others = [c["ref"] for c in cases], [c["other"] for c in cases]
+    flows = [f.cpu().numpy() for f in batch.solve(refs, others)]
+    again = [f.cpu().numpy() for f in batch.solve(refs, others)]
+    for (h, w), c, f, f2 in zip(sizes, cases, flows, again):
+        print(f"pair {h}x{w}", flush=True)
+        np.testing.assert_array_equal(f, f2)
+        p = copy.copy(prm)
+        p.height, p.width = h, w
+        v = V.VFS(p)
+        v.solver = "blocked"
+        v.pd_tiles = tiles
+        v.setup_module()
+        v.compute_pyramid_sizes()
+        v.create_mask_pyramid(c["mask"])
+        v.create_vector_field_pyramid(c["tvf"], c["cvf"])
+        np.testing.assert_array_equal(f, v.solve_stereo(c["ref"], c["other"]).cpu().numpy())
+        assert np.all(f[c["mask"] == 0] == 0)
