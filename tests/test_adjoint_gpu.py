@@ -272,10 +272,11 @@ def test_gradients_do_not_depend_on_checkpoints_or_kernel_choice(T):
 
 
 @pytest.mark.parametrize("shape,ckpt,rows", [((54, 70), 4, None), ((48, 48), 9, None), ((54, 70), 4, 1),
-                                             ((54, 70), 5, 2), ((54, 70), 3, 4), ((100, 68), 4, 4)])
+                                             ((54, 70), 5, 2), ((54, 70), 3, 4), ((100, 68), 4, 4), ((17, 23), 4, 1)])
 def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt, rows):
     """The fused reverse sweep (dual^T + primal^T + threshold^T per launch, ring messages in
-    LDS) against the one-kernel-per-step sweep; partial tiles at the right / bottom edges."""
+    LDS) against the one-kernel-per-step sweep; partial tiles at the right / bottom edges
+    (17 x 23: every tile partial, levels of 17 x 23 and 8 x 11)."""
     import synthetic as S
     import vfs_adjoint as A
     import vfs_params as P
@@ -306,7 +307,8 @@ def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt, rows):
 
 @pytest.mark.parametrize("shape,ckpt,rb", [((54, 70), 4, 2), ((48, 48), 9, 3), ((54, 70), 5, 4), ((61, 47), 4, 1),
                                            ((54, 70), 9, (2, 512)), ((37, 90), 4, (1, 512)), ((61, 47), 7, 5),
-                                           ((54, 70), 9, 6), ((37, 90), 9, (3, 512))])
+                                           ((54, 70), 9, 6), ((37, 90), 9, (3, 512)), ((17, 23), 4, 3),
+                                           ((17, 23), 9, (2, 512))])
 def test_blocked_sweep_matches_fused_sweep(T, shape, ckpt, rb):
     """The temporally blocked reverse sweep (K units per launch, adjoint state in registers,
     messages in LDS) against the fused sweep.  The per-pixel adjoints are the same arithmetic
