@@ -1,0 +1,43 @@
+"""The rocprofv3 summariser that writes the kernel tables kept with the measurements:
+per-kernel aggregation of a kernel trace, and the per-launch-shape table of the
+blocked kernel, on a small hand-made trace."""
+
+import csv
+
+import rocprof_summary as R
+
+FIELDS = ["Kind", "Kernel_Name", "Start_Timestamp", "End_Timestamp", "Grid_Size_X"]
+
+
+def _trace(path, rows):
+    path.mkdir()
+    with open(path / "t_kernel_trace.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=FIELDS)
+        w.writeheader()
+        for name, t0, t1, grid in rows:
+            w.writerow(dict(Kind="KERNEL_DISPATCH", Kernel_Name=name, Start_Timestamp=t0, End_Timestamp=t1,
+                            Grid_Size_X=grid))
+
+
+def test_short_name_strips_namespace_and_arguments():
+    n = "void (anonymous namespace)::k_pd_block<22, 22, 5, 1024>(PDLevel, int, PDSched, PDConsts)"
+    assert R.short_name(n) == "k_pd_block<22, 22, 5, 1024>"
+    assert R.short_name("k_copy(float*, float const*, int)") == "k_copy"
+
+
+def test_stats_aggregates_durations_and_launch_shapes(tmp_path):
+    pd = "void (anonymous namespace)::k_pd_block<22, 22, 5, 1024>(PDLevel, int, PDSched, PDConsts)"
+    rows = [(pd, 1000, 31000, 1401856), (pd, 40000, 74000, 1401856), (pd, 80000, 90000, 409600),
+            ("k_derivatives(float2*, float const*)", 100000, 105000, 832)]
+    d = tmp_path / "trace"
+    _trace(d, rows)
+    md = R.stats(str(d), "T", "CMD")
+    lines = md.splitlines()
+    assert lines[0] == "# T" and "CMD" in lines
+    row = next(l for l in lines if l.startswith("| k_pd_block<22, 22, 5, 1024> | 3 |"))
+    cells = [c.strip() for c in row.strip("|").split("|")]
+    # 30 + 34 + 10 us over 3 calls; min 10, max 34; 74 of 79 us in total
+    assert cells[1:] == ["3", "74", "24.667", "10.000", "34.000", "93.67"]
+    assert "| k_pd_block<22, 22, 5, 1024> | 1401856 | 2 | 32.000 |" in lines
+    assert "| k_pd_block<22, 22, 5, 1024> | 409600 | 1 | 10.000 |" in lines
+    assert lines[-1] == "All k_pd_block launches: 3 calls, average 24.667 us."
