@@ -1,6 +1,7 @@
-"""The rocprofv3 summariser that writes the kernel tables and HBM traffic kept with the measurements:
-per-kernel aggregation of a kernel trace, and the per-launch-shape table of the
-blocked kernel, on a small hand-made trace."""
+"""The rocprofv3 summariser that writes the kernel tables and HBM traffic kept with the
+measurements, on small hand-made traces: per-kernel aggregation and the blocked kernel's
+per-launch-shape table of a kernel trace, and the copy-scaled FETCH_SIZE / WRITE_SIZE
+bytes per k_pd_block launch."""
 
 import csv
 
