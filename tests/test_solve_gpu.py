@@ -84,6 +84,23 @@ def test_solver_modes_agree_bitwise(T, small_case):
     np.testing.assert_array_equal(outs["graph"], outs["python"])
 
 
+@pytest.mark.parametrize("mode", ["graph", "blocked"])
+def test_each_solve_returns_its_own_flow(T, small_case, mode):
+    """solve_stereo returns a new tensor every call, as the reference returns a new array
+    (Python/vfs.py:462-533): keeping the first result across a second solve of other
+    images leaves it unchanged."""
+    pair = small_case["pair"]
+    v = _solver(small_case, mode)
+    first = v.solve_stereo(pair["img_ref"], pair["img_other"])
+    kept = first.clone()
+    second = v.solve_stereo(pair["img_other"], pair["img_ref"])
+    T.cuda.synchronize()
+    assert first.data_ptr() != second.data_ptr()
+    assert T.equal(first, kept)
+    assert not T.equal(first, second)
+    assert v.warpUV is second
+
+
 def test_graph_replays_track_new_inputs(T, small_case):
     """Replays read the new images: eager solves interleaved with replays agree every time."""
     pair = small_case["pair"]

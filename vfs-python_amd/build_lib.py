@@ -4,6 +4,15 @@ Usage: python build_lib.py [--arch gfx950] [--out PATH]
 
 All HIP sources under csrc/ are compiled into one shared library next to this
 file.  Packed-FP32 instruction selection is disabled for every HIP source.
+
+Floating-point contraction is limited to single expressions (-ffp-contract=on)
+instead of hipcc's default of fusing a multiply into an add across statements.
+The solve's paths (per-kernel, blocked, batched) and the adjoint's sweeps inline
+the same per-pixel functions into different kernels; with cross-statement fusion
+the compiler may fuse a product into a later addition in one kernel and not in
+another, so the paths could differ in the last bit.  Within one expression the
+choice is made from the source alone, so every kernel rounds each shared
+function the same way.
 """
 
 import argparse
@@ -28,7 +37,7 @@ def build(arch="gfx950", out=None, verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = out + ".tmp"
     cmd = [hipcc, f"--offload-arch={arch}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on",
            "-Wall", "-Wno-unused-function",
            "-I", os.path.join(HERE, "csrc"), "-I", os.path.join(HERE, "..", "include"),
            *srcs, "-o", tmp]

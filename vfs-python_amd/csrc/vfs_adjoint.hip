@@ -4,11 +4,13 @@
 // iteration is reversed step by step (primal, threshold, dual), re-reading the
 // forward state of the iteration from a tape that the host recomputes from
 // checkpoints.  Contributions to a neighbour's adjoint are written as per-pixel
-// messages and collected by the receiving pixel ("pull"), so these sums are
-// deterministic; the bilinear transposes (warp, derivative taps, pyramid,
-// upsampling) scatter with float atomics.  Gradients of the scalar parameters
-// (lambda, alpha0, alpha1, beta, gamma) accumulate per pixel in a (H, W, 5)
-// plane and are reduced once at the end (vfs_adj_reduce).
+// messages and collected by the receiving pixel ("pull").  The transposes of the
+// fixed stencils and bilinear taps (derivative taps, tensor, Gaussian, pyramid,
+// upsampling) are gathers too: each receiving pixel sums its senders' shares in a
+// fixed order.  No kernel here uses atomics, so the gradients are the same bits on
+// every run.  Gradients of the scalar parameters (lambda, alpha0, alpha1, beta,
+// gamma) accumulate per pixel in a (H, W, 5) plane, in the order of the reversed
+// steps, and are reduced once at the end (vfs_reduce_sum).
 //
 // Non-smooth points follow one convention throughout: projections are active
 // when the norm exceeds 1; max(exp, 1e-8) differentiates exp when exp >= 1e-8;
@@ -275,40 +277,60 @@ __global__ void __launch_bounds__(256) k_adj_median(float* __restrict__ gu_th, c
     gu_th[idx] = acc;
 }
 
-__device__ __forceinline__ void scatter_bilinear(float* __restrict__ g, int H, int W, float x, float y, float v)
+// Contribution of one bilinear tap (weight v at texel coordinate (x, y)) to the
+// adjoint of texel r: the tap's corner weights of every corner that is r, in the
+// order i00, i10, i01, i11.  A non-finite coordinate or a zero v contributes
+// nothing.  The transposes below are gathers built on this: each receiving texel
+// recomputes the taps of every sender that can reach it and adds their shares in
+// a fixed order, so the sums do not depend on scheduling.
+__device__ __forceinline__ float tap_share(int r, int H, int W, float x, float y, float v)
 {
-    if (v == 0.0f || !(x == x) || !(y == y)) return;
+    if (v == 0.0f || !(x == x) || !(y == y)) return 0.0f;
     const Tap t = make_tap(x, y, H, W);
-    atomicAdd(&g[t.i00], v * (1.0f - t.ax) * (1.0f - t.ay));
-    atomicAdd(&g[t.i10], v * t.ax * (1.0f - t.ay));
-    atomicAdd(&g[t.i01], v * (1.0f - t.ax) * t.ay);
-    atomicAdd(&g[t.i11], v * t.ax * t.ay);
+    float acc = 0.0f;
+    if (t.i00 == r) acc += v * (1.0f - t.ax) * (1.0f - t.ay);
+    if (t.i10 == r) acc += v * t.ax * (1.0f - t.ay);
+    if (t.i01 == r) acc += v * (1.0f - t.ax) * t.ay;
+    if (t.i11 == r) acc += v * t.ax * t.ay;
+    return acc;
 }
 
-// Reverse of computeDerivatives (Python/vfs.cu:120-142): scatters the adjoint of
-// (Iu, Iz) onto the reference level image and the warped image (atomics).
+// Reverse of computeDerivatives (Python/vfs.cu:120-142).  The sender at pixel s
+// weights its 5-point taps s + k e (k = -2, -1, 1, 2; e the unit tvf at s) of both
+// the reference level image and the warped image by giu * (1, -8, 8, -1), giu =
+// g(Iu) / 24, and its centre by -g(Iz) (reference) and +g(Iz) (warped).  |k e| <= 2,
+// so a tap's corners lie within [s - 2, s + 3] per axis and, after the mirror,
+// every sender of texel r lies within r +- 3.  Gather: gref += T - g(Iz),
+// gow += T + g(Iz), with T the taps' shares at r summed over the 7 x 7 window in
+// row-major order.
 __global__ void __launch_bounds__(256) k_adj_derivatives(float* __restrict__ gref, float* __restrict__ gow,
                                                          const float* __restrict__ gderiv,
                                                          const float2* __restrict__ tvf, int H, int W)
 {
     VFS_ADJ_PIXEL(H, W)
-    const float giu = gderiv[2 * idx] / 24.0f, giz = gderiv[2 * idx + 1];
-    if (giz != 0.0f) {
-        atomicAdd(&gow[idx], giz);
-        atomicAdd(&gref[idx], -giz);
+    float acc = 0.0f;
+    for (int dy = -3; dy <= 3; ++dy) {
+        const int sy = Y + dy;
+        if (sy < 0 || sy >= H) continue;
+        for (int dx = -3; dx <= 3; ++dx) {
+            const int sx = X + dx;
+            if (sx < 0 || sx >= W) continue;
+            const int s = sy * W + sx;
+            const float giu = gderiv[2 * s] / 24.0f;
+            if (giu == 0.0f) continue;
+            const float2 vec = tvf[s];
+            const float nrm = hypotf(vec.x, vec.y);
+            const float ex = vec.x / nrm, ey = vec.y / nrm;
+            const float x = static_cast<float>(sx), y = static_cast<float>(sy);
+            acc += tap_share(idx, H, W, fmaf(-2.0f, ex, x), fmaf(-2.0f, ey, y), giu * 1.0f);
+            acc += tap_share(idx, H, W, x - ex, y - ey, giu * -8.0f);
+            acc += tap_share(idx, H, W, x + ex, y + ey, giu * 8.0f);
+            acc += tap_share(idx, H, W, fmaf(2.0f, ex, x), fmaf(2.0f, ey, y), giu * -1.0f);
+        }
     }
-    if (giu == 0.0f) return;
-    const float2 vec = tvf[idx];
-    const float nrm = hypotf(vec.x, vec.y);
-    const float ex = vec.x / nrm, ey = vec.y / nrm;
-    const float x = static_cast<float>(X), y = static_cast<float>(Y);
-    const float xs[4] = {fmaf(-2.0f, ex, x), x - ex, x + ex, fmaf(2.0f, ex, x)};
-    const float ys[4] = {fmaf(-2.0f, ey, y), y - ey, y + ey, fmaf(2.0f, ey, y)};
-    const float wt[4] = {1.0f, -8.0f, 8.0f, -1.0f};
-    for (int k = 0; k < 4; ++k) {
-        scatter_bilinear(gref, H, W, xs[k], ys[k], giu * wt[k]);
-        scatter_bilinear(gow, H, W, xs[k], ys[k], giu * wt[k]);
-    }
+    const float giz = gderiv[2 * idx + 1];
+    gow[idx] += acc + giz;
+    gref[idx] += acc - giz;
 }
 
 // derivative of a bilinear sample with respect to its coordinates
@@ -360,16 +382,16 @@ __global__ void __launch_bounds__(256) k_adj_warp(float2* __restrict__ gf, const
 // per-level reverse: tensor, Gaussian, pyramid, upsampling
 // ---------------------------------------------------------------------------
 
-// Reverse of computeTensorEtaMasked (Python/vfs.cu:228-276): the adjoints of abc and
-// eta go to the smoothed image (atomics on its three taps) and to beta, gamma,
-// alpha0, alpha1 (per-pixel accumulators).
-__global__ void __launch_bounds__(256) k_adj_tensor(float* __restrict__ gimg, float* __restrict__ gpar,
-                                                    const float* __restrict__ gabc, const float* __restrict__ geta,
-                                                    const float* __restrict__ img, const uint8_t* __restrict__ mask,
-                                                    int H, int W, float beta, float gamma, float alpha0, float alpha1)
+// Reverse of computeTensorEtaMasked (Python/vfs.cu:228-276) at the masked-in pixel
+// (X, Y): the adjoints of its abc and eta give the adjoints (g_gx, g_gy) of its
+// forward differences gx = I(X+1,Y+1) - I(X,Y+1), gy = I(X+1,Y+1) - I(X+1,Y) and,
+// when gpp is not null, add its shares of beta, gamma, alpha0, alpha1 to gpp.
+__device__ __forceinline__ float2 tensor_adj_pixel(int X, int Y, int H, int W, const float* __restrict__ gabc,
+                                                   const float* __restrict__ geta, const float* __restrict__ img,
+                                                   float beta, float gamma, float alpha0, float alpha1,
+                                                   float* __restrict__ gpp)
 {
-    VFS_ADJ_PIXEL(H, W)
-    if (mask[idx] == 0) return;
+    const int idx = Y * W + X;
     const int xp = clamp_idx(X + 1, W), yp = clamp_idx(Y + 1, H);
     const int i11 = yp * W + xp, i01 = yp * W + X, i10 = Y * W + xp;
     const float gx = img[i11] - img[i01];
@@ -390,21 +412,22 @@ __global__ void __launch_bounds__(256) k_adj_tensor(float* __restrict__ gimg, fl
     ga += ge0 * a11 * (2.0f * a + 2.0f * (a + c)) + ge2 * 2.0f * a11 * a;
     gb += ge0 * a11 * (2.0f * b + 2.0f * (b + c)) + ge1 * 2.0f * a11 * b;
     gc += ge0 * a11 * (4.0f * c + 2.0f * (a + c) + 2.0f * (b + c)) + ge1 * 2.0f * a11 * c + ge2 * 2.0f * a11 * c;
-    const float g_a11 = ge0 * (a * a + b * b + 2.0f * c * c + (a + c) * (a + c) + (b + c) * (b + c)) +
-                        ge1 * (b * b + c * c) + ge2 * (a * a + c * c);
-    float* gpp = gpar + GP_N * idx;
-    gpp[GP_ALPHA1] += g_a11 * 2.0f * alpha1;
-    gpp[GP_ALPHA0] += (ge1 + ge2) * 8.0f * alpha0;
+    if (gpp != nullptr) {
+        const float g_a11 = ge0 * (a * a + b * b + 2.0f * c * c + (a + c) * (a + c) + (b + c) * (b + c)) +
+                            ge1 * (b * b + c * c) + ge2 * (a * a + c * c);
+        gpp[GP_ALPHA1] += g_a11 * 2.0f * alpha1;
+        gpp[GP_ALPHA0] += (ge1 + ge2) * 8.0f * alpha0;
+    }
     const float g_w = ga * nx * nx + gb * ny * ny + gc * nx * ny;
     float g_nx = ga * 2.0f * w * nx + gb * 2.0f * nx + gc * (w - 1.0f) * ny;
     float g_ny = ga * 2.0f * ny + gb * 2.0f * w * ny + gc * (w - 1.0f) * nx;
     float g_ng = 0.0f;
     if (wact) {
         const float g_P = -g_w * ex * beta;
-        gpp[GP_BETA] += -g_w * ex * P;
+        if (gpp != nullptr) gpp[GP_BETA] += -g_w * ex * P;
         if (ng > 0.0f) {
             g_ng += g_P * gamma * P / ng;
-            gpp[GP_GAMMA] += g_P * P * logf(ng);
+            if (gpp != nullptr) gpp[GP_GAMMA] += g_P * P * logf(ng);
         }
     }
     float g_gx = 0.0f, g_gy = 0.0f;
@@ -417,67 +440,152 @@ __global__ void __launch_bounds__(256) k_adj_tensor(float* __restrict__ gimg, fl
         g_gx += g_ng * gx / ng;
         g_gy += g_ng * gy / ng;
     }
-    atomicAdd(&gimg[i11], g_gx + g_gy);
-    atomicAdd(&gimg[i01], -g_gx);
-    atomicAdd(&gimg[i10], -g_gy);
+    return make_float2(g_gx, g_gy);
 }
 
-// Reverse of gaussianMasked (Python/vfs.cu:81-102,204-226).
+// Reverse of computeTensorEtaMasked: every masked-in pixel s sends g_gx + g_gy to
+// i11(s) = (clamp(Y+1), clamp(X+1)), -g_gx to i01(s) = (clamp(Y+1), X) and -g_gy to
+// i10(s) = (Y, clamp(X+1)), so the senders of texel r are the pixels r - (0 or 1,
+// 0 or 1).  Gather: each texel adds the shares of its up to four senders (row-major,
+// then i11, i01, i10), recomputing theirs; its own pass also adds its parameter
+// shares to gpar.
+__global__ void __launch_bounds__(256) k_adj_tensor(float* __restrict__ gimg, float* __restrict__ gpar,
+                                                    const float* __restrict__ gabc, const float* __restrict__ geta,
+                                                    const float* __restrict__ img, const uint8_t* __restrict__ mask,
+                                                    int H, int W, float beta, float gamma, float alpha0, float alpha1)
+{
+    VFS_ADJ_PIXEL(H, W)
+    float acc = 0.0f;
+    for (int dy = -1; dy <= 0; ++dy) {
+        const int sy = Y + dy;
+        if (sy < 0) continue;
+        for (int dx = -1; dx <= 0; ++dx) {
+            const int sx = X + dx;
+            if (sx < 0) continue;
+            const int s = sy * W + sx;
+            if (mask[s] == 0) continue;
+            const bool own = dx == 0 && dy == 0;
+            const float2 g = tensor_adj_pixel(sx, sy, H, W, gabc, geta, img, beta, gamma, alpha0, alpha1,
+                                              own ? gpar + GP_N * idx : nullptr);
+            const int xp = clamp_idx(sx + 1, W), yp = clamp_idx(sy + 1, H);
+            if (yp * W + xp == idx) acc += g.x + g.y;
+            if (yp * W + sx == idx) acc += -g.x;
+            if (sy * W + xp == idx) acc += -g.y;
+        }
+    }
+    gimg[idx] += acc;
+}
+
+// Reverse of gaussianMasked (Python/vfs.cu:81-102,204-226): every masked-in pixel s
+// sends (1, 2, 1)^T (1, 2, 1) / 16 of its adjoint to its clamped 3 x 3 neighbourhood.
+// Gather over the 3 x 3 senders of each texel (row-major), each sender's taps in the
+// order centre, corners, edges.
 __global__ void __launch_bounds__(256) k_adj_gaussian(float* __restrict__ gimg, const float* __restrict__ gout,
                                                       const uint8_t* __restrict__ mask, int H, int W)
 {
     VFS_ADJ_PIXEL(H, W)
-    if (mask[idx] == 0) return;
-    const float g = gout[idx];
-    if (g == 0.0f) return;
-    const int xm = clamp_idx(X - 1, W), xp = clamp_idx(X + 1, W);
-    const int ym = clamp_idx(Y - 1, H), yp = clamp_idx(Y + 1, H);
-    atomicAdd(&gimg[idx], 0.25f * g);
-    const float g1 = 0.0625f * g, g2 = 0.125f * g;
-    atomicAdd(&gimg[ym * W + xm], g1);
-    atomicAdd(&gimg[yp * W + xm], g1);
-    atomicAdd(&gimg[ym * W + xp], g1);
-    atomicAdd(&gimg[yp * W + xp], g1);
-    atomicAdd(&gimg[Y * W + xm], g2);
-    atomicAdd(&gimg[Y * W + xp], g2);
-    atomicAdd(&gimg[ym * W + X], g2);
-    atomicAdd(&gimg[yp * W + X], g2);
+    float acc = 0.0f;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int sy = Y + dy;
+        if (sy < 0 || sy >= H) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int sx = X + dx;
+            if (sx < 0 || sx >= W) continue;
+            const int s = sy * W + sx;
+            if (mask[s] == 0) continue;
+            const float g = gout[s];
+            if (g == 0.0f) continue;
+            const int xm = clamp_idx(sx - 1, W), xp = clamp_idx(sx + 1, W);
+            const int ym = clamp_idx(sy - 1, H), yp = clamp_idx(sy + 1, H);
+            const float g1 = 0.0625f * g, g2 = 0.125f * g;
+            if (s == idx) acc += 0.25f * g;
+            if (ym * W + xm == idx) acc += g1;
+            if (yp * W + xm == idx) acc += g1;
+            if (ym * W + xp == idx) acc += g1;
+            if (yp * W + xp == idx) acc += g1;
+            if (sy * W + xm == idx) acc += g2;
+            if (sy * W + xp == idx) acc += g2;
+            if (ym * W + sx == idx) acc += g2;
+            if (yp * W + sx == idx) acc += g2;
+        }
+    }
+    gimg[idx] += acc;
 }
 
-// Reverse of downSamplingImage (four bilinear taps, Python/vfs.cu:35-49,171-174).
+// Senders s in [0, n_send) of a resampling whose sample coordinate is
+// t = fmaf(s + off, ratio, -0.5) with off in [off_lo, off_hi] and whose corners can
+// fall on texel r.  Corners are floor(t) and floor(t) + 1, within [-1, n] for these
+// resamplings, and the mirror takes -1 to 0 and n to n - 1, so |t - r| <= 1:
+// s + off in [(r - 0.5) / ratio, (r + 1.5) / ratio].  One more sender on each side
+// absorbs the rounding of the bounds.
+__device__ __forceinline__ void sender_range(int r, int n_send, float ratio, float off_lo, float off_hi, int& lo,
+                                             int& hi)
+{
+    lo = static_cast<int>(floorf((static_cast<float>(r) - 0.5f) / ratio - off_hi)) - 1;
+    hi = static_cast<int>(ceilf((static_cast<float>(r) + 1.5f) / ratio - off_lo)) + 1;
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > n_send - 1 ? n_send - 1 : hi;
+}
+
+// Reverse of downSamplingImage (four bilinear taps, Python/vfs.cu:35-49,171-174):
+// the coarse pixel s sends a quarter of its adjoint through each of its taps
+// (s + 1/4, s + 1/2), (s + 3/4, s + 1/2), (s + 1/2, s + 1/4), (s + 1/2, s + 3/4) (in
+// coarse pixels, scaled by the size ratio).  Gather over the fine texel's senders
+// (row-major, taps in that order).
 __global__ void __launch_bounds__(256) k_adj_downsample(float* __restrict__ gsrc, int Hs, int Ws,
                                                         const float* __restrict__ gout, int Hd, int Wd, float rx,
                                                         float ry)
 {
-    VFS_ADJ_PIXEL(Hd, Wd)
-    const float g = 0.25f * gout[idx];
-    if (g == 0.0f) return;
-    const float xc = fmaf(X + 0.5f, rx, -0.5f), yc = fmaf(Y + 0.5f, ry, -0.5f);
-    const float xm = fmaf(X + 0.25f, rx, -0.5f), xp = fmaf(X + 0.75f, rx, -0.5f);
-    const float ym = fmaf(Y + 0.25f, ry, -0.5f), yp = fmaf(Y + 0.75f, ry, -0.5f);
-    scatter_bilinear(gsrc, Hs, Ws, xm, yc, g);
-    scatter_bilinear(gsrc, Hs, Ws, xp, yc, g);
-    scatter_bilinear(gsrc, Hs, Ws, xc, ym, g);
-    scatter_bilinear(gsrc, Hs, Ws, xc, yp, g);
+    VFS_ADJ_PIXEL(Hs, Ws)
+    int xlo, xhi, ylo, yhi;
+    sender_range(X, Wd, rx, 0.25f, 0.75f, xlo, xhi);
+    sender_range(Y, Hd, ry, 0.25f, 0.75f, ylo, yhi);
+    float acc = 0.0f;
+    for (int sy = ylo; sy <= yhi; ++sy) {
+        for (int sx = xlo; sx <= xhi; ++sx) {
+            const float g = 0.25f * gout[sy * Wd + sx];
+            if (g == 0.0f) continue;
+            const float xc = fmaf(sx + 0.5f, rx, -0.5f), yc = fmaf(sy + 0.5f, ry, -0.5f);
+            const float xm = fmaf(sx + 0.25f, rx, -0.5f), xp = fmaf(sx + 0.75f, rx, -0.5f);
+            const float ym = fmaf(sy + 0.25f, ry, -0.5f), yp = fmaf(sy + 0.75f, ry, -0.5f);
+            acc += tap_share(idx, Hs, Ws, xm, yc, g);
+            acc += tap_share(idx, Hs, Ws, xp, yc, g);
+            acc += tap_share(idx, Hs, Ws, xc, ym, g);
+            acc += tap_share(idx, Hs, Ws, xc, yp, g);
+        }
+    }
+    gsrc[idx] += acc;
 }
 
 // Reverse of upSamplingVectorField(1) (Python/vfs.cu:69-79,678-705) for `channels`
-// interleaved channels: gsrc += scale * S^T gout.
+// interleaved channels: gsrc += scale * S^T gout.  The fine pixel s samples the
+// coarse level at (s + 1/2) (Ws / Wd) - 1/2; gather over the coarse texel's senders
+// (row-major).
 __global__ void __launch_bounds__(256) k_adj_upsample(float* __restrict__ gsrc, int Hs, int Ws,
                                                       const float* __restrict__ gout, int Hd, int Wd, int channels,
                                                       float rx, float ry, float scale)
 {
-    VFS_ADJ_PIXEL(Hd, Wd)
-    const float xs = fmaf(X + 0.5f, rx, -0.5f), ys = fmaf(Y + 0.5f, ry, -0.5f);
-    const Tap t = make_tap(xs, ys, Hs, Ws);
-    for (int ch = 0; ch < channels; ++ch) {
-        const float g = scale * gout[channels * idx + ch];
-        if (g == 0.0f) continue;
-        atomicAdd(&gsrc[channels * t.i00 + ch], g * (1.0f - t.ax) * (1.0f - t.ay));
-        atomicAdd(&gsrc[channels * t.i10 + ch], g * t.ax * (1.0f - t.ay));
-        atomicAdd(&gsrc[channels * t.i01 + ch], g * (1.0f - t.ax) * t.ay);
-        atomicAdd(&gsrc[channels * t.i11 + ch], g * t.ax * t.ay);
+    VFS_ADJ_PIXEL(Hs, Ws)
+    int xlo, xhi, ylo, yhi;
+    sender_range(X, Wd, rx, 0.5f, 0.5f, xlo, xhi);
+    sender_range(Y, Hd, ry, 0.5f, 0.5f, ylo, yhi);
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int sy = ylo; sy <= yhi; ++sy) {
+        for (int sx = xlo; sx <= xhi; ++sx) {
+            const Tap t = make_tap(fmaf(sx + 0.5f, rx, -0.5f), fmaf(sy + 0.5f, ry, -0.5f), Hs, Ws);
+            if (t.i00 != idx && t.i10 != idx && t.i01 != idx && t.i11 != idx) continue;
+            const int s = sy * Wd + sx;
+            for (int ch = 0; ch < channels; ++ch) {
+                const float g = scale * gout[channels * s + ch];
+                if (g == 0.0f) continue;
+                if (t.i00 == idx) acc[ch] += g * (1.0f - t.ax) * (1.0f - t.ay);
+                if (t.i10 == idx) acc[ch] += g * t.ax * (1.0f - t.ay);
+                if (t.i01 == idx) acc[ch] += g * (1.0f - t.ax) * t.ay;
+                if (t.i11 == idx) acc[ch] += g * t.ax * t.ay;
+            }
+        }
     }
+    for (int ch = 0; ch < channels; ++ch) gsrc[channels * idx + ch] += acc[ch];
 }
 
 // ---------------------------------------------------------------------------
@@ -736,7 +844,7 @@ int vfs_adj_downsample(void* stream, float* gsrc, int Hs, int Ws, const float* g
     VFS_CHECK_DIMS(Hd, Wd);
     VFS_CHECK_PTR(gsrc);
     VFS_CHECK_PTR(gout);
-    hipLaunchKernelGGL(k_adj_downsample, grid2d(Hd, Wd), block2d(), 0, S(stream), gsrc, Hs, Ws, gout, Hd, Wd,
+    hipLaunchKernelGGL(k_adj_downsample, grid2d(Hs, Ws), block2d(), 0, S(stream), gsrc, Hs, Ws, gout, Hd, Wd,
                        static_cast<float>(Ws) / static_cast<float>(Wd), static_cast<float>(Hs) / static_cast<float>(Hd));
     VFS_LAUNCH_STATUS();
 }
@@ -749,7 +857,7 @@ int vfs_adj_upsample(void* stream, float* gsrc, int Hs, int Ws, const float* gou
     VFS_CHECK_PTR(gsrc);
     VFS_CHECK_PTR(gout);
     if (channels < 1 || channels > 4) return VFS_EINVAL;
-    hipLaunchKernelGGL(k_adj_upsample, grid2d(Hd, Wd), block2d(), 0, S(stream), gsrc, Hs, Ws, gout, Hd, Wd, channels,
+    hipLaunchKernelGGL(k_adj_upsample, grid2d(Hs, Ws), block2d(), 0, S(stream), gsrc, Hs, Ws, gout, Hd, Wd, channels,
                        static_cast<float>(Ws) / static_cast<float>(Wd), static_cast<float>(Hs) / static_cast<float>(Hd),
                        f_scale);
     VFS_LAUNCH_STATUS();

@@ -76,34 +76,23 @@ struct AdjWs {
     float *gp2, *gq2, *qmsg2, *tmsg2;
     // second copies of the state the blocked sweep also reads across tiles
     float *gu2, *gub2, *gvb2;
-    double* gpart;    // fused sweep: (lambda, alpha0, alpha1) per 32 x 8 block
-    double* gpart_b;  // blocked sweep: the same per tile
     float* tape;
 };
 
-// fused sweep tile width and the most per-block parameter slots it can need
-// (tiles of 32 x 8, one row per thread)
+// fused sweep tile (32 x 8, one row per thread)
 constexpr int FT_X = 32, FT_Y = 8;
-inline int64_t fused_blocks(int H, int W)
-{
-    return static_cast<int64_t>((W + FT_X - 1) / FT_X) * ((H + FT_Y - 1) / FT_Y);
-}
 
-// Upper bound of blocked-sweep tiles (every reverse tile is at least 8 x 8).
-inline int64_t rev_tiles_max(int H, int W) { return static_cast<int64_t>((W + 7) / 8) * ((H + 7) / 8); }
-
-inline int64_t adj_ws_floats(int64_t n, int64_t nblocks, int64_t ntiles)
+inline int64_t adj_ws_floats(int64_t n)
 {
     // tp 2, gp 2, gq 4, um 3, vm 6, qmsg 4, tmsg 4, gabc 3, geta 3, gderiv 2, gvbB 2, gubB 1, guf 1, gtv 2,
     // gow 1, guth 1, gvb_fin 2, gIs 1, pos 1, the second copies gp 2, gq 4, qmsg 4, tmsg 4, gu 1, gubB 1, gvbB 2
     const int ch[] = {2, 2, 4, 3, 6, 4, 4, 3, 3, 2, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 4, 4, 4, 1, 1, 2};
     int64_t t = 0;
     for (int c : ch) t += al(c * n);
-    // + (lambda, alpha0, alpha1) per block and per tile in double
-    return t + al(2 * 3 * nblocks) + al(2 * 3 * ntiles);
+    return t;
 }
 
-inline AdjWs adj_ws_at(float* base, int64_t n, int64_t nblocks, int64_t ntiles)
+inline AdjWs adj_ws_at(float* base, int64_t n)
 {
     AdjWs a;
     float* c = base;
@@ -138,10 +127,6 @@ inline AdjWs adj_ws_at(float* base, int64_t n, int64_t nblocks, int64_t ntiles)
     a.gu2 = take(1);
     a.gub2 = take(1);
     a.gvb2 = take(2);
-    a.gpart = reinterpret_cast<double*>(c);
-    c += al(2 * 3 * nblocks);
-    a.gpart_b = reinterpret_cast<double*>(c);
-    c += al(2 * 3 * ntiles);
     a.tape = c;
     return a;
 }
@@ -290,8 +275,9 @@ __global__ void __launch_bounds__(256) k_add(float* __restrict__ out, const floa
 // The head of a sweep is DO_PL alone (iteration n - 1, no messages yet), the tail
 // DO_D + FINISH (iteration 0).  Per-pixel arithmetic is that of the one-kernel-
 // per-step path; the primal^T / threshold^T messages travel in compact form
-// (qmsg = (-g_S1, -g_S2), tmsg = g_g); lambda / alpha shares are summed per
-// block into double slots.
+// (qmsg = (-g_S1, -g_S2), tmsg = g_g).  Each pixel adds its lambda / alpha shares
+// to its own slots of gpar in the order of the reversed steps, as the one-kernel-
+// per-step path does, so the two sweeps give the same bits.
 struct FusedArgs {
     float* gu;
     float* gubB;
@@ -307,7 +293,7 @@ struct FusedArgs {
     float* gabc;
     float* geta;
     float2* gderiv;
-    double* gpart;
+    float* gpar;  // (H, W, 5) per-pixel parameter shares
     float* gub_out;
     const float2* p1;  // S_i
     const float4* q1;
@@ -384,14 +370,13 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
 {
     constexpr int TX = FT_X, TY = 8 * R, LX = TX + 2, LY = TY + 2, RING = 2 * LX + 2 * TY;
     __shared__ float s_msg[DO_D ? LY * LX * 9 : 1];
-    __shared__ float s_red[3 * 4];
     const int tid = threadIdx.x, tx = tid & (TX - 1), ty = tid / TX;
     int bx, by;
     tile_of(blockIdx.x, a.nbx, a.nby, bx, by);
     const int X0 = bx * TX, Y0 = by * TY;
     const int H = a.H, W = a.W;
     const int X = X0 + tx;
-    float gpl = 0.0f, gpa0 = 0.0f, gpa1 = 0.0f;
+    float gpl[R], gpa0[R], gpa1[R];
     float2 gpo[R];
     float4 gqo[R];
     float gab[R][3];
@@ -401,9 +386,14 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
         gpo[r] = make_float2(0.0f, 0.0f);
         gqo[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         for (int k = 0; k < 3; ++k) gab[r][k] = 0.0f;
+        gpl[r] = gpa0[r] = gpa1[r] = 0.0f;
         if (X < W && Y < H) {
             const int idx = Y * W + X;
             for (int k = 0; k < 3; ++k) gab[r][k] = a.gabc[3 * idx + k];
+            const float* gp = a.gpar + GP_N * idx;
+            gpl[r] = gp[GP_LAMBDA];
+            gpa0[r] = gp[GP_ALPHA0];
+            gpa1[r] = gp[GP_ALPHA1];
         }
     }
     if constexpr (DO_D) {
@@ -412,7 +402,7 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
             const int Y = Y0 + ty + 8 * r;
             if (X < W && Y < H) {
                 DualMsg m;
-                fused_dual(a, X, Y, gpo[r], gqo[r], m, gab[r], gpa0, gpa1);
+                fused_dual(a, X, Y, gpo[r], gqo[r], m, gab[r], gpa0[r], gpa1[r]);
                 put_msg(s_msg, (ty + 8 * r + 1) * LX + tx + 1, m);
             }
         }
@@ -474,11 +464,12 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
                 const float ea = a.abc[3 * idx], eb = a.abc[3 * idx + 1], ec = a.abc[3 * idx + 2];
                 primal_adj_pixel(n, a.q1[idx], n.mW ? a.q1[n.iW] : zero4, n.mN ? a.q1[n.iN] : zero4, ea, eb, ec,
                                  a.p1[idx], a.eta[3 * idx + 1], a.eta[3 * idx + 2], a.tau_pl, a.mu_pl, a.alpha0,
-                                 a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_, qm, gab[r], e1, e2, gpa0, gpa1, qmc);
+                                 a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_, qm, gab[r], e1, e2, gpa0[r], gpa1[r],
+                                 qmc);
                 float2 gd = a.gderiv[idx];
                 gu_ = l1_adj_pixel(n, gu_, tp_at(a.p1, a.abc, a.mask, idx), tp_at(a.p1, a.abc, a.mask, n.iW).x,
                                    tp_at(a.p1, a.abc, a.mask, n.iN).y, a.deriv[idx], a.eta[3 * idx], a.u2[idx],
-                                   a.ub2[idx], a.tau_pl, a.lambda, tm, ubB, e0, gd, gpl, g_g);
+                                   a.ub2[idx], a.tau_pl, a.lambda, tm, ubB, e0, gd, gpl[r], g_g);
                 a.gu[idx] = gu_;
                 a.gubB[idx] = ubB;
                 a.gvbB[idx] = vbB;
@@ -491,12 +482,21 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
                 a.geta[3 * idx + 2] = e2;
                 a.gderiv[idx] = gd;
                 for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[r][k];
+                float* gp = a.gpar + GP_N * idx;
+                gp[GP_LAMBDA] = gpl[r];
+                gp[GP_ALPHA0] = gpa0[r];
+                gp[GP_ALPHA1] = gpa1[r];
             }
         } else {
             // DO_D alone: the dual^T outputs of the owned pixel
             a.gp_out[idx] = gpo[r];
             a.gq_out[idx] = gqo[r];
             for (int k = 0; k < 3; ++k) a.gabc[3 * idx + k] = gab[r][k];
+            if (a.mask[idx] != 0) {
+                float* gp = a.gpar + GP_N * idx;
+                gp[GP_ALPHA0] = gpa0[r];
+                gp[GP_ALPHA1] = gpa1[r];
+            }
             if constexpr (FINISH) {
                 float g = a.gubB[idx];
                 float2 gv = a.gvbB[idx];
@@ -505,34 +505,6 @@ __global__ void __launch_bounds__(256) k_adj_fused(const FusedArgs a)
             }
         }
     }
-    // block sums of the lambda / alpha shares (wavefront shuffles, then four wavefronts)
-    float v[3] = {gpl, gpa0, gpa1};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-    if ((tid & 63) == 0)
-        for (int k = 0; k < 3; ++k) s_red[3 * (tid >> 6) + k] = v[k];
-    __syncthreads();
-    if (tid == 0) {
-        double* slot = a.gpart + 3 * (static_cast<int64_t>(by) * a.nbx + bx);
-        for (int k = 0; k < 3; ++k)
-            slot[k] += static_cast<double>(s_red[k]) + static_cast<double>(s_red[3 + k]) +
-                       static_cast<double>(s_red[6 + k]) + static_cast<double>(s_red[9 + k]);
-    }
-}
-
-// Adds each fused tile's (lambda, alpha0, alpha1) sums to the per-pixel shares of
-// the tile's first pixel.
-__global__ void __launch_bounds__(256) k_fold_gpart(float* __restrict__ gpar, const double* __restrict__ gpart,
-                                                    int W, int nbx, int tile_y, int nblocks)
-{
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    const int X = (b % nbx) * FT_X, Y = (b / nbx) * tile_y;
-    float* g = gpar + static_cast<int64_t>(GP_N) * (static_cast<int64_t>(Y) * W + X);
-    g[GP_LAMBDA] += static_cast<float>(gpart[3 * b]);
-    g[GP_ALPHA0] += static_cast<float>(gpart[3 * b + 1]);
-    g[GP_ALPHA1] += static_cast<float>(gpart[3 * b + 2]);
 }
 
 // Rows per thread of the fused sweep at a level of n pixels.  Measured on MI355X at
@@ -581,9 +553,9 @@ int launch_fused_rows(int rows, int kind, int nblocks, hipStream_t s, const Fuse
 // g v_bar, gp, gq) and the accumulators of abc, eta and the derivatives stay in
 // registers; the messages and, per unit, the neighbours' forward state go through LDS
 // (each pixel loads its own state from the tape).  Per-pixel arithmetic is the
-// fused kernel's (vfs_adj_pixel.h), in the same order, so the tile's values equal
-// the fused sweep's bit for bit; only the per-tile (lambda, alpha0, alpha1) sums are
-// grouped differently.
+// fused kernel's (vfs_adj_pixel.h), in the same order, and each tile pixel keeps its
+// (lambda, alpha0, alpha1) shares in registers that start from and return to its
+// gpar slots, so the tile's values equal the fused sweep's bit for bit.
 // ---------------------------------------------------------------------------
 
 constexpr int KRMAX = 6;
@@ -608,7 +580,7 @@ struct BlockRevArgs {
     float* gabc;
     float* geta;
     float2* gderiv;
-    double* gpart;  // (lambda, alpha0, alpha1) per tile
+    float* gpar;  // (H, W, 5) per-pixel parameter shares
     const float* abc;
     const float* eta;
     const float2* deriv;
@@ -683,7 +655,6 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
     // q (4); two buffers, alternating between units
     __shared__ float sFS[2][9 * E];
     __shared__ unsigned sFL[E];
-    __shared__ float s_red[3 * (NT / 64)];
     const int tid = threadIdx.x;
     const int H = a.H, W = a.W;
     const int t = xcd_item(blockIdx.x, gridDim.x);
@@ -693,9 +664,9 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
     float gu[S], gub[S], gvx[S], gvy[S], gpx[S], gpy[S], gqx[S], gqy[S], gqz[S], gqw[S];
     float ab[S][3], et[S][3], dd[S][2];   // accumulators (tile pixels)
     float ca[S], cb[S], cc[S], e0[S], e1[S], e2[S], iu[S], iz[S];
+    float gpl[S], gpa0[S], gpa1[S];       // parameter shares (tile pixels)
     int gi[S];
     unsigned fl[S];
-    float gpl = 0.0f, gpa0 = 0.0f, gpa1 = 0.0f;
 
 #pragma unroll
     for (int j = 0; j < S; ++j) {
@@ -707,6 +678,7 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
         ca[j] = cb[j] = cc[j] = e0[j] = e1[j] = e2[j] = iu[j] = iz[j] = 0.0f;
         for (int c = 0; c < 3; ++c) ab[j][c] = et[j][c] = 0.0f;
         dd[j][0] = dd[j][1] = 0.0f;
+        gpl[j] = gpa0[j] = gpa1[j] = 0.0f;
         float qm0 = 0.0f, qm1 = 0.0f, tm0 = 0.0f;
         if (e < E && gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const int idx = gy * W + gx;
@@ -741,6 +713,10 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
                     const float2 g = a.gderiv[idx];
                     dd[j][0] = g.x;
                     dd[j][1] = g.y;
+                    const float* gp = a.gpar + GP_N * idx;
+                    gpl[j] = gp[GP_LAMBDA];
+                    gpa0[j] = gp[GP_ALPHA0];
+                    gpa1[j] = gp[GP_ALPHA1];
                 }
             }
             gu[j] = a.gu_in[idx];
@@ -873,18 +849,13 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
                 const int eS = (gy + 1 < H && ey + 1 < EY) ? e + EX : e, eN = (gy > 0 && ey > 0) ? e - EX : e;
                 const int ix = fx ? eE : eW, iy = fy ? eS : eN;
                 // the per-pixel accumulators run at every pixel of the region (only the tile's
-                // are stored), the per-tile parameter sums at the tile's pixels only
-                float pa0 = 0.0f, pa1 = 0.0f;
+                // are stored)
                 dual_adj_pixel(fx, fy, ub1[j], make_float2(vb1x[j], vb1y[j]), fsb[2 * E + ix],
                                make_float2(fsb[3 * E + ix], fsb[4 * E + ix]), fsb[2 * E + iy],
                                make_float2(fsb[3 * E + iy], fsb[4 * E + iy]), ca[j], cb[j], cc[j],
                                make_float2(p1x[j], p1y[j]),
                                make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), a.st[u], make_float2(gpx[j], gpy[j]), gqn,
-                               gtp, gpo, gqo, m, ab[j], pa0, pa1);
-                if (f & RB_TILE) {
-                    gpa0 += pa0;
-                    gpa1 += pa1;
-                }
+                               gtp, gpo, gqo, m, ab[j], gpa0[j], gpa1[j]);
             }
             gpx[j] = gpo.x;
             gpy[j] = gpo.y;
@@ -943,7 +914,6 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             n.mN = (f & RB_MN) != 0;
             // W / N neighbours (the pixel itself at the image border, or beyond the region)
             const int eW = (gx > 0 && ex > 0) ? e - 1 : e, eN = (gy > 0 && ey > 0) ? e - EX : e;
-            float pa0 = 0.0f, pa1 = 0.0f, pl = 0.0f;
             float2 gd = make_float2(dd[j][0], dd[j][1]);
             float ubB, g_g;
             float2 vbB, qmc;
@@ -958,17 +928,12 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             primal_adj_pixel(n, make_float4(q1x[j], q1y[j], q1z[j], q1w[j]), qW, qN, ca[j], cb[j], cc[j],
                              make_float2(p1x[j], p1y[j]), e1[j],
                              e2[j], a.tau_pl[u], a.mu_pl[u], a.alpha0, a.alpha1, g_ubo, g_vbo, ubB, vbB, gu_, gp_, gq_,
-                             qm, ab[j], et[j][1], et[j][2], pa0, pa1, qmc);
+                             qm, ab[j], et[j][1], et[j][2], gpa0[j], gpa1[j], qmc);
             gu_ = l1_adj_pixel(n, gu_, make_float2(fsb[e], fsb[E + e]), fsb[eW], fsb[E + eN], make_float2(iu[j], iz[j]),
                                e0[j], u2[j], ub2[j], a.tau_pl[u], a.lambda, tm, ubB, et[j][0], gd,
-                               pl, g_g);
+                               gpl[j], g_g);
             dd[j][0] = gd.x;
             dd[j][1] = gd.y;
-            if (f & RB_TILE) {
-                gpa0 += pa0;
-                gpa1 += pa1;
-                gpl += pl;
-            }
             gu[j] = gu_;
             gub[j] = ubB;
             gvx[j] = vbB.x;
@@ -1005,38 +970,12 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
                 a.geta[3 * idx + c] = et[j][c];
             }
             a.gderiv[idx] = make_float2(dd[j][0], dd[j][1]);
+            float* gp = a.gpar + GP_N * idx;
+            gp[GP_LAMBDA] = gpl[j];
+            gp[GP_ALPHA0] = gpa0[j];
+            gp[GP_ALPHA1] = gpa1[j];
         }
     }
-    // tile sums of the lambda / alpha shares (wavefront shuffles, then the wavefronts)
-    float v[3] = {gpl, gpa0, gpa1};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-    if ((tid & 63) == 0)
-        for (int k = 0; k < 3; ++k) s_red[3 * (tid >> 6) + k] = v[k];
-    __syncthreads();
-    if (tid == 0) {
-        double* slot = a.gpart + 3 * (static_cast<int64_t>(bty) * a.tiles_x + btx);
-        for (int k = 0; k < 3; ++k) {
-            double acc = 0.0;
-            for (int w = 0; w < NT / 64; ++w) acc += static_cast<double>(s_red[3 * w + k]);
-            slot[k] += acc;
-        }
-    }
-}
-
-// Adds each blocked tile's (lambda, alpha0, alpha1) sums to the per-pixel shares of
-// the tile's first pixel.
-__global__ void __launch_bounds__(256) k_fold_gpart_tiles(float* __restrict__ gpar, const double* __restrict__ gpart,
-                                                          int W, int tiles_x, int tile_x, int tile_y, int ntiles)
-{
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= ntiles) return;
-    const int X = (b % tiles_x) * tile_x, Y = (b / tiles_x) * tile_y;
-    float* g = gpar + static_cast<int64_t>(GP_N) * (static_cast<int64_t>(Y) * W + X);
-    g[GP_LAMBDA] += static_cast<float>(gpart[3 * b]);
-    g[GP_ALPHA0] += static_cast<float>(gpart[3 * b + 1]);
-    g[GP_ALPHA1] += static_cast<float>(gpart[3 * b + 2]);
 }
 
 // Reverse variants (sweep modes): 20 + KR (1-6 units) with 1024 threads on a 32 x 32
@@ -1134,7 +1073,7 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt)
 {
     if (H <= 0 || W <= 0 || ckpt < 1) return 0;
     const int64_t n = static_cast<int64_t>(H) * W;
-    return static_cast<size_t>(adj_ws_floats(n, fused_blocks(H, W), rev_tiles_max(H, W)) +
+    return static_cast<size_t>(adj_ws_floats(n) +
                                static_cast<int64_t>(ckpt) * state_floats(n)) *
            sizeof(float);
 }
@@ -1155,7 +1094,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
     if (fk > 0) {
         // blocked path: k-iteration launches between two state sets, with a launch
         // boundary at every checkpoint; set 1 is the first tape slot of the workspace
-        const AdjWs A = adj_ws_at(workspace, n, fused_blocks(H, W), rev_tiles_max(H, W));
+        const AdjWs A = adj_ws_at(workspace, n);
         const State t1 = state_at(A.tape, n);
         float* su[2] = {lv->u0, t1.u};
         float* sub[2] = {lv->u_bar, t1.ub};
@@ -1262,20 +1201,14 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     const int64_t n = static_cast<int64_t>(H) * W;
     hipStream_t s = S(stream);
     float* sv = const_cast<float*>(save);
-    const int nbx = (W + FT_X - 1) / FT_X, nby = (H + FT_Y - 1) / FT_Y;
-    const int64_t nblocks = static_cast<int64_t>(nbx) * nby;
-    const AdjWs A = adj_ws_at(workspace, n, nblocks, rev_tiles_max(H, W));
+    const int nbx = (W + FT_X - 1) / FT_X;
+    const AdjWs A = adj_ws_at(workspace, n);
     const int nck = n_ckpt(n_iter, ckpt);
     VFS_TRY(vfs_fill(stream, A.gabc, 0.0f, 3 * n));
     VFS_TRY(vfs_fill(stream, A.geta, 0.0f, 3 * n));
-    VFS_TRY(vfs_fill(stream, reinterpret_cast<float*>(A.gpart), 0.0f, 2 * 3 * nblocks));
-    // blocked sweep tiles (rv): the tile grid and its (lambda, alpha0, alpha1) slots
+    // blocked sweep tiles (rv)
     const int rtiles_x = rv ? (W + rv->tx - 1) / rv->tx : 0, rtiles_y = rv ? (H + rv->ty - 1) / rv->ty : 0;
     const int rtiles = rtiles_x * rtiles_y;
-    if (rv != nullptr) {
-        if (rtiles > rev_tiles_max(H, W)) return VFS_EINVAL;
-        VFS_TRY(vfs_fill(stream, reinterpret_cast<float*>(A.gpart_b), 0.0f, 2 * 3 * static_cast<int64_t>(rtiles)));
-    }
     FusedArgs fa;
     fa.gu = A.guth;
     fa.gubB = A.gubB;
@@ -1283,7 +1216,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     fa.gabc = A.gabc;
     fa.geta = A.geta;
     fa.gderiv = reinterpret_cast<float2*>(A.gderiv);
-    fa.gpart = A.gpart;
+    fa.gpar = gpar;
     fa.gub_out = gub;
     fa.abc = lv->abc;
     fa.eta = lv->eta;
@@ -1459,7 +1392,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                     ba.gabc = A.gabc;
                     ba.geta = A.geta;
                     ba.gderiv = reinterpret_cast<float2*>(A.gderiv);
-                    ba.gpart = A.gpart_b;
+                    ba.gpar = gpar;
                     ba.abc = lv->abc;
                     ba.eta = lv->eta;
                     ba.deriv = reinterpret_cast<const float2*>(lv->deriv);
@@ -1557,16 +1490,6 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         VFS_TRY(vfs_fill(stream, gu, 0.0f, n));
         VFS_TRY(vfs_fill(stream, gub, 0.0f, n));
         VFS_TRY(vfs_fill(stream, gf, 0.0f, 2 * n));
-    }
-    if (n_warp > 0 && nck > 0 && fused) {
-        hipLaunchKernelGGL(k_fold_gpart, dim3(nb1(fblocks)), dim3(256), 0, s, gpar, A.gpart, W, nbx, tile_y,
-                           fblocks);
-        if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
-        if (rv != nullptr) {
-            hipLaunchKernelGGL(k_fold_gpart_tiles, dim3(nb1(rtiles)), dim3(256), 0, s, gpar, A.gpart_b, W, rtiles_x,
-                               rv->tx, rv->ty, rtiles);
-            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
-        }
     }
     VFS_TRY(vfs_fill(stream, A.gIs, 0.0f, n));
     VFS_TRY(vfs_adj_tensor(stream, A.gIs, gpar, A.gabc, A.geta, lv->ref_smoothed, lv->mask, H, W, prm));

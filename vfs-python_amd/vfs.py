@@ -341,7 +341,9 @@ class VFS:
         self.refs, self.others = b["refs"], b["others"]
         self.refs_to = [texture.wrap(x) for x in b["refs"]]
         self.others_to = [texture.wrap(x) for x in b["others"]]
-        self.warpUV = b["result"]
+        # a fresh tensor per call, as the reference's solve returns a new array
+        # (Python/vfs.py:462-533); the level buffers are reused by the next solve
+        self.warpUV = b["result"].clone()
         return self.warpUV
 
     def release_plan(self):

@@ -105,6 +105,9 @@ class VFSDepthAdjoint:
         self.s = None
         self._b = None
         self._done_forward = False
+        # forwards run so far: DepthLoss checks that a loss's backward reads the state
+        # of its own forward
+        self.generation = 0
         # optional shared workspace: a dict that instances solved one after another on the
         # same stream may share ({"ws": uint8 device tensor}, grown on demand)
         self._shared_ws = workspace
@@ -251,6 +254,7 @@ class VFSDepthAdjoint:
         self.n_valid = max(float(sums[1]), 1.0)
         self.loss = float(sums[0]) / self.n_valid
         self._done_forward = True
+        self.generation += 1
         return self.loss
 
     # -- reverse --------------------------------------------------------------------------------
@@ -298,18 +302,32 @@ class VFSDepthAdjoint:
 
 
 class DepthLoss(torch.autograd.Function):
-    """loss = DepthLoss.apply(ref0, theta, adj, img_other); theta = (lambda, alpha0, alpha1, beta, gamma)."""
+    """loss = DepthLoss.apply(ref0, theta, adj, img_other); theta = (lambda, alpha0, alpha1, beta, gamma).
+
+    The reverse sweep reads the state that ``adj`` keeps from its latest forward, so a
+    loss can be differentiated only while it is that latest forward of ``adj``: a
+    second DepthLoss.apply on the same ``adj`` before the first loss's backward makes
+    that backward raise instead of returning the second loss's gradient.  Use one
+    VFSDepthAdjoint per loss that is kept for backward."""
 
     @staticmethod
     def forward(ctx, ref0, theta, adj, img_other):
         adj.set_params(theta.detach().cpu().numpy())
         loss = adj.forward(ref0.detach(), img_other)
         ctx.adj = adj
-        return torch.tensor(loss, dtype=torch.float64)
+        ctx.generation = adj.generation
+        ctx.theta_meta = (theta.device, theta.dtype)
+        ctx.ref0_meta = (ref0.device, ref0.dtype)
+        return torch.tensor(loss, dtype=torch.float64, device=theta.device)
 
     @staticmethod
     def backward(ctx, gl):
+        if ctx.adj.generation != ctx.generation:
+            raise RuntimeError("DepthLoss.backward: the VFSDepthAdjoint ran another forward after this loss; "
+                               "its saved state belongs to that forward")
         g = ctx.adj.backward()
         gl = float(gl)
-        gtheta = torch.tensor([g[n] * gl for n in PARAMS], dtype=torch.float64)
-        return g["ref0"] * gl, gtheta, None, None
+        dev, dt = ctx.theta_meta
+        gtheta = torch.tensor([g[n] * gl for n in PARAMS], dtype=dt, device=dev)
+        rdev, rdt = ctx.ref0_meta
+        return (g["ref0"] * gl).to(device=rdev, dtype=rdt), gtheta, None, None
