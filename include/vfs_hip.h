@@ -352,22 +352,23 @@ int vfs_adj_clamp(void* stream, float* gu, float* guf, int* pos, float* gtv, con
                   float limit, int ksize);
 /* Reverse of medianFilter: gu_th (H,W) := adjoint of the thresholded u. */
 int vfs_adj_median(void* stream, float* gu_th, const float* guf, const int* pos, int H, int W, int ksize);
-/* Reverse of computeDerivatives (Python/vfs.cu:120-142): gref, gow (H,W) += (atomics). */
+/* Reverse of computeDerivatives (Python/vfs.cu:120-142): gref, gow (H,W) += (gather: each
+ * texel sums its senders' tap shares in a fixed order, so the result is deterministic). */
 int vfs_adj_derivatives(void* stream, float* gref, float* gow, const float* gderiv, const float* tvf,
                         int H, int W);
 /* Reverse of warpingImage / findWarpingVector at the warp-start flow:
  * gf (H,W,2) += gow dS(other)/dx + gtv . dS(tvf)/dx. */
 int vfs_adj_warp(void* stream, float* gf, const float* gow, const float* gtv, const float* flow,
                  const float* other, const float* tvf, int H, int W);
-/* Reverse of computeTensorEtaMasked (Python/vfs.cu:228-276): gimg (H,W) += (atomics);
+/* Reverse of computeTensorEtaMasked (Python/vfs.cu:228-276): gimg (H,W) += (gather);
  * gpar (H,W,5) += beta, gamma, alpha0, alpha1 shares. */
 int vfs_adj_tensor(void* stream, float* gimg, float* gpar, const float* gabc, const float* geta,
                    const float* img, const uint8_t* mask, int H, int W, const vfs_params* prm);
-/* Reverse of gaussianMasked: gimg (H,W) += (atomics). */
+/* Reverse of gaussianMasked: gimg (H,W) += (gather). */
 int vfs_adj_gaussian(void* stream, float* gimg, const float* gout, const uint8_t* mask, int H, int W);
-/* Reverse of downSamplingImage: gsrc (Hs,Ws) += (atomics). */
+/* Reverse of downSamplingImage: gsrc (Hs,Ws) += (gather). */
 int vfs_adj_downsample(void* stream, float* gsrc, int Hs, int Ws, const float* gout, int Hd, int Wd);
-/* Reverse of upSamplingVectorField(1): gsrc (Hs,Ws,channels) += f_scale S^T gout (atomics). */
+/* Reverse of upSamplingVectorField(1): gsrc (Hs,Ws,channels) += f_scale S^T gout (gather). */
 int vfs_adj_upsample(void* stream, float* gsrc, int Hs, int Ws, const float* gout, int Hd, int Wd,
                      int channels, float f_scale);
 /* Depth loss |log z - log z_gt| per pixel from the flow (z of triangulateRays,
