@@ -32,6 +32,7 @@ extern "C" {
 #endif
 
 #define VFS_EINVAL (-22)
+#define VFS_ETOOBIG (-7) /* a size beyond a fixed capacity of the library */
 
 /* Solver parameters: runtime values of the constants that Python/vfs.py:52-68
  * substitutes into the CUDA source before NVRTC compilation. */
@@ -389,6 +390,18 @@ int vfs_reduce_sum(void* stream, double* out, double* partial, int nparts, const
  * block); partial holds 2 * nparts doubles (1 <= nparts <= 65535). */
 int vfs_residual_norm(void* stream, double* out, double* partial, int nparts, const float* a, const float* b,
                       const uint8_t* mask, int64_t n, int channels);
+
+/* vfs_residual_norm without the caller's `partial` workspace: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It allocates
+ * nothing: the partials live in a static device array of VFS_NOALLOC_MAX_PARTS pairs
+ * of doubles (16 KiB) that each GPU holds until the process ends.  nparts beyond that
+ * maximum returns VFS_ETOOBIG; the result is the same bits as vfs_residual_norm with
+ * the same nparts.  `stream` must belong to the current device.  Calls on one GPU
+ * share its array, so calls that overlap on one GPU (other streams or threads) race. */
+#define VFS_NOALLOC_MAX_PARTS 1024
+int vfs_residual_norm_noalloc(void* stream, double* out, int nparts, const float* a, const float* b,
+                              const uint8_t* mask, int64_t n, int channels);
 
 /* ---- adjoint level driver ---------------------------------------------------------
  * The forward warp loop of one level that records what the reverse needs, and the

@@ -16,6 +16,8 @@
 // when the norm exceeds 1; max(exp, 1e-8) differentiates exp when exp >= 1e-8;
 // min(d, limit) passes the gradient to d when d < limit; the threshold uses
 // the branch its forward took; the median routes to the selected element.
+#include <mutex>
+
 #include "vfs_adj_pixel.h"
 #include "vfs_median.h"
 
@@ -724,6 +726,41 @@ inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace
 
+// Scratch of vfs_residual_norm_noalloc: the per-workgroup partials (sum of squares,
+// pixel count) of at most VFS_NOALLOC_MAX_PARTS workgroups, 16 KiB.  Every device the
+// library runs on holds its own copy until the process ends.  Host code never passes
+// this name to a kernel: it is a host-side stand-in for the symbol, and the device
+// address comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) double scratch[2 * VFS_NOALLOC_MAX_PARTS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch`, resolved once per device
+// (the first call on that device) and kept.
+int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return static_cast<int>(e);
+    if (dev < 0 || dev >= kMaxDevices) return VFS_EINVAL;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return static_cast<int>(e);
+        if (p == nullptr) return VFS_EINVAL;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int vfs_adj_pd_iteration(void* stream, const vfs_adj_pd* a, const vfs_params* prm, float tau, float sigma,
@@ -891,6 +928,21 @@ int vfs_residual_norm(void* stream, double* out, double* partial, int nparts, co
         return VFS_EINVAL;
     hipLaunchKernelGGL(k_residual_partial, dim3(nparts), dim3(256), 0, S(stream), partial, a, b, mask, n, channels);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, S(stream), out, partial, nparts, 2);
+    VFS_LAUNCH_STATUS();
+}
+
+int vfs_residual_norm_noalloc(void* stream, double* out, int nparts, const float* a, const float* b,
+                              const uint8_t* mask, int64_t n, int channels)
+{
+    if (out == nullptr || a == nullptr || b == nullptr || n < 0 || channels < 1 || nparts < 1) return VFS_EINVAL;
+    if (nparts > VFS_NOALLOC_MAX_PARTS) return VFS_ETOOBIG;
+    double* part = nullptr;
+    const int rc = noalloc_scratch(&part);
+    if (rc != 0) return rc;
+    // every partial slot the final block reads (0 .. nparts - 1) is written by this
+    // launch's first kernel, so the scratch needs no clearing
+    hipLaunchKernelGGL(k_residual_partial, dim3(nparts), dim3(256), 0, S(stream), part, a, b, mask, n, channels);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, S(stream), out, part, nparts, 2);
     VFS_LAUNCH_STATUS();
 }
 

@@ -15,6 +15,8 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvfs_hip.so")
 VFS_EINVAL = -22
+VFS_ETOOBIG = -7
+VFS_NOALLOC_MAX_PARTS = 1024
 
 _lib = None
 
@@ -119,6 +121,7 @@ _SIGS = {
     "vfs_depth_loss": [c_p] * 6 + [c_i, c_i, c_f, c_f, c_f, c_f, c_f],
     "vfs_reduce_sum": [c_p, c_p, c_p, c_i, c_p, c_i64, c_i],
     "vfs_residual_norm": [c_p, c_p, c_p, c_i, c_p, c_p, c_p, c_i64, c_i],
+    "vfs_residual_norm_noalloc": [c_p, c_p, c_i, c_p, c_p, c_p, c_i64, c_i],
     "vfs_adj_forward_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
                               c_p, c_p, ctypes.POINTER(VfsAdjOpts)],
     "vfs_adj_reverse_level": [c_p, ctypes.POINTER(VfsLevel), ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i,
@@ -168,6 +171,8 @@ def check(rc, name):
     if rc != 0:
         if rc == VFS_EINVAL:
             raise RuntimeError(f"{name}: invalid argument (sizes or null pointer)")
+        if rc == VFS_ETOOBIG:
+            raise RuntimeError(f"{name}: size beyond the library's fixed capacity")
         raise RuntimeError(f"{name}: HIP error {rc}")
 
 
