@@ -300,8 +300,10 @@ def solve_stereo(ref0, other0_warped, masks, trs, sizes, lam, alpha0, alpha1, be
 
 
 def depth_loss(flow, cam, t, m, z_gt):
-    """mean |log z - log z_gt| over mask pixels with z > 0."""
+    """mean |log z - log z_gt| over the valid mask pixels: z finite and > 0, z_gt > 0
+    (the pixels k_depth_loss counts)."""
     z = triangulate_z(flow, cam, t, m)
-    ok = m & (z > 0)
+    ok = m & (z > 0) & torch.isfinite(z) & (z_gt > 0)
     zs = torch.where(ok, z, torch.ones_like(z))
-    return torch.sum(torch.where(ok, torch.abs(torch.log(zs) - torch.log(z_gt)), torch.zeros_like(z))) / ok.sum()
+    zg = torch.where(ok, z_gt, torch.ones_like(z_gt))
+    return torch.sum(torch.where(ok, torch.abs(torch.log(zs) - torch.log(zg)), torch.zeros_like(z))) / ok.sum()

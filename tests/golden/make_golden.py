@@ -4,9 +4,15 @@ Fixtures (inputs and expected outputs only, compressed .npz):
   kernels_24x32.npz  seeded random inputs of every per-kernel function at 24x32
                      with a ragged mask, and the oracle's outputs;
   solve_64.npz       a seeded 64x64 synthetic fisheye pair, float32 vector fields,
-                     and the oracle's flow after 2 levels x 3 warps x 10 iterations.
+                     and the oracle's flow after 2 levels x 3 warps x 10 iterations;
+  solve_800.npz      the 800x800 workload (seed 0: 5 levels x 20 warps x 50
+                     iterations, the reference's default solver parameters): the
+                     uint8 pair, mask and pose, and, on every 4th row and column, the
+                     float64 oracle's flow and depth, the ground-truth depth and the
+                     float32 restatement's flow (the spread that single-precision
+                     rounding alone gives the solve).  About 4 minutes of CPU.
 
-Run from the repository root: python tests/golden/make_golden.py
+Run from the repository root: python tests/golden/make_golden.py [--only solve_800]
 """
 
 import os
@@ -90,10 +96,54 @@ def solve_case(seed=5, n=64):
                 schedule=np.array([2, 3, 10], np.int64))
 
 
+SUB = 4  # stride of the stored rows and columns of the 800x800 results
+
+
+def _solve_800(args):
+    """One oracle solve of the 800x800 workload in float64 or float32 (run in a pool)."""
+    dtype, pair = args
+    w = S.WORKLOADS["plain_full_800"]
+    cam = pair["cam"]
+    hh, ww = pair["img_ref"].shape
+    tvf, cvf = O.get_vector_fields(O.get_grid_zero((hh, ww)), cam, cam, pair["R"], pair["t"], 1)
+    tvf = S.sanitize_field(tvf)
+    cvf = cvf.astype(np.float32)
+    prm = O.OracleParams(hh, ww)
+    S.apply_workload_params(prm, w)
+    if dtype == "f32":
+        with O.precision(np.float32):
+            return O.solve_stereo(pair["img_ref"], pair["img_other"], pair["mask"], tvf, cvf, prm)
+    return O.solve_stereo(pair["img_ref"], pair["img_other"], pair["mask"], tvf, cvf, prm)
+
+
+def solve_800(seed=0):
+    from multiprocessing import Pool
+    pair = S.workload_pair("plain_full_800", seed=seed)
+    with Pool(2) as pool:
+        f64, f32 = pool.map(_solve_800, [("f64", pair), ("f32", pair)])
+    xyz = O.triangulate_flow(f64, pair["cam"], pair["t"].reshape(3), pair["mask"])
+    m = pair["mask"] > 0
+    z, zg = xyz[..., 2], pair["z_gt"]
+    ok = m & np.isfinite(z) & (zg > 1e-3)
+    err = float(np.median(np.abs(z[ok] - zg[ok]) / zg[ok]))
+    print(f"solve_800: oracle median relative depth error against the ground truth {err:.4f}")
+    sub = (slice(None, None, SUB), slice(None, None, SUB))
+    return dict(img_ref=pair["img_ref"], img_other=pair["img_other"], mask=pair["mask"], cam=np.array(pair["cam"]),
+                R=pair["R"], t=pair["t"].reshape(3), seed=np.int64(seed), sub=np.int64(SUB),
+                flow64=f64[sub].astype(np.float32), flow32=f32[sub].astype(np.float32),
+                z64=z[sub].astype(np.float32), z_gt=zg[sub].astype(np.float32),
+                oracle_err_vs_gt=np.float64(err))
+
+
 def main():
-    a = kernel_inputs()
-    np.savez_compressed(os.path.join(HERE, "kernels_24x32.npz"), **a, **kernel_outputs(a))
-    np.savez_compressed(os.path.join(HERE, "solve_64.npz"), **solve_case())
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only in (None, "kernels_24x32"):
+        a = kernel_inputs()
+        np.savez_compressed(os.path.join(HERE, "kernels_24x32.npz"), **a, **kernel_outputs(a))
+    if only in (None, "solve_64"):
+        np.savez_compressed(os.path.join(HERE, "solve_64.npz"), **solve_case())
+    if only in (None, "solve_800"):
+        np.savez_compressed(os.path.join(HERE, "solve_800.npz"), **solve_800())
     print("wrote", sorted(os.listdir(HERE)))
 
 

@@ -5,8 +5,13 @@ and central finite differences of the float64 NumPy oracle.
 Tolerances (fp32 state, fp32 adjoint accumulation, against float64 references):
   * N reversed primal-dual iterations: every input adjoint within 1e-4 of the
     autograd gradient relative to the largest entry (same fp32 inputs);
-  * full solve, parameters and reference image: cosine similarity >= 0.999 and
-    relative error <= 2e-2 per parameter against autograd through the float64 solve;
+  * full solve (48 x 48, 2 levels x 2 warps x 6 iterations), parameters and reference
+    image: relative error <= 1e-3 per parameter (measured 5e-4 to 8.7e-4) and cosine
+    similarity >= 0.999 against autograd through the float64 solve;
+  * the differentiation workload's schedule (3 levels x 10 warps x 50 iterations) on
+    a 128 x 128 crop of its 800 x 800 pair: against central differences of the
+    float64 oracle at h = 1e-5 (relative), the gradient vector within 1e-3 relative
+    (measured 7e-4); see the test for the finite differences' own convergence;
   * finite differences (central, h = 1e-4 relative) of the float64 oracle loss:
     per parameter within max(2e-2 relative, 3x the deviation of the same gradient
     taken by autograd through the restatement run in float32).  The depth loss
@@ -169,12 +174,14 @@ def test_solve_adjoint_matches_autograd_through_unrolled_torch(T):
     flow = R.solve_stereo(r0, ow, masks, trs, sizes, *th, n_warp, n_iter)
     lt = R.depth_loss(flow, pr["cam"], torch.as_tensor(pr["t"].reshape(3)), masks[0], torch.as_tensor(pr["z_gt"]))
     lt.backward()
-    print("loss hip", loss, "torch", float(lt))
-    assert abs(loss - float(lt)) <= 1e-3 * abs(float(lt))
+    print("loss hip", loss, "torch", float(lt.detach()))
+    assert np.isfinite(float(lt.detach()))
+    assert abs(loss - float(lt.detach())) <= 1e-3 * abs(float(lt.detach()))
     for i, k in enumerate(("Lambda", "alpha0", "alpha1", "beta", "gamma")):
         ref = th[i].grad.item()
-        print(k, g[k], ref)
-        assert abs(g[k] - ref) <= 2e-2 * abs(ref) + 1e-6, k
+        print(k, g[k], ref, "relative", abs(g[k] - ref) / abs(ref))
+        # measured 5.0e-4 to 8.7e-4 relative
+        assert abs(g[k] - ref) <= 1e-3 * abs(ref), k
     gr, rr = g["ref0"].cpu().numpy().ravel(), r0.grad.numpy().ravel()
     cos = float(np.dot(gr, rr) / (np.linalg.norm(gr) * np.linalg.norm(rr)))
     print("ref0 cosine", cos, "rel", np.linalg.norm(gr - rr) / np.linalg.norm(rr))
@@ -186,7 +193,7 @@ def _oracle_loss(pr, tvf, cvf, theta, n_warp, n_iter, levels=2):
     prm = O.OracleParams(n, n, nLevel=levels, fScale=2.0, nWarpIters=n_warp, nSolverIters=n_iter, **theta)
     flow = O.solve_stereo(pr["img_ref"], pr["img_other"], pr["mask"], tvf, cvf, prm)
     z = O.triangulate_flow(flow, pr["cam"], pr["t"].reshape(3), pr["mask"])[..., 2]
-    m = (pr["mask"] > 0) & (z > 0) & np.isfinite(z)
+    m = (pr["mask"] > 0) & (z > 0) & np.isfinite(z) & (pr["z_gt"] > 0)
     return float(np.mean(np.abs(np.log(z[m]) - np.log(pr["z_gt"][m]))))
 
 
@@ -457,3 +464,75 @@ def test_depth_loss_autograd_function(T, theta_device):
         la.backward()
     lb.backward()
     assert torch.isfinite(r1.grad).all()
+
+
+def test_adjoint_matches_central_differences_on_a_crop_of_the_800_workload(T):
+    """The differentiation workload's schedule (3 levels x 10 warps x 50 iterations,
+    checkpoint 25, the reference's default parameters) on the 128 x 128 centre crop of
+    its seed-0 800 x 800 pair (camera principal point moved with the crop, mask with a
+    3-pixel zero border).
+
+    Central differences of the float64 oracle's loss converge as h shrinks, since the
+    solve is piecewise smooth (threshold branches, projections, median): relative to
+    the float64 autograd of the restatement they are off by up to 1.9 % at h = 1e-3,
+    1.2 % at 1e-4 and 0.14 % at 1e-5 (measured).  At h = 1e-5:
+      * the differences agree with the float64 autograd: gradient vector within 1e-3
+        relative (measured 4e-4) -- the checker is exact;
+      * the hand adjoint (fp32) agrees with the differences: gradient vector within
+        1e-3 relative (measured 7e-4; per parameter lambda 2e-4, alpha1 7e-4, beta
+        9e-5, gamma 6e-4, and alpha0, whose derivative is 6e-5 of the vector's norm,
+        4e-3)."""
+    import synthetic as S
+    import vfs_adjoint as A
+    import vfs_params as P
+    pr = S.workload_pair("adjoint_800", seed=0)
+    c0, n = 400 - 64, 128
+    sl = (slice(c0, c0 + n), slice(c0, c0 + n))
+    f, cx, cy = pr["cam"]
+    cam = (f, cx - c0, cy - c0)
+    ref, oth, zg = pr["img_ref"][sl].copy(), pr["img_other"][sl].copy(), pr["z_gt"][sl].copy()
+    mask = np.zeros((n, n), np.uint8)
+    mask[3:-3, 3:-3] = 255
+    tvf, cvf = O.get_vector_fields(O.get_grid_zero((n, n)), cam, cam, pr["R"], pr["t"], 1)
+    tvf, cvf = S.sanitize_field(tvf), cvf.astype(np.float32)
+    w = S.WORKLOADS["adjoint_800"]
+    nl, nw, ni = w["nLevel"], w["nWarpIters"], w["nSolverIters"]
+    theta = {k: float(w[k]) for k in A.PARAMS}
+    prm = P.vfs_params(ref)
+    S.apply_workload_params(prm, w)
+    adj = A.VFSDepthAdjoint(prm, mask, tvf, cvf, cam, pr["t"], zg, checkpoint=w["checkpoint"])
+    adj.forward(torch.as_tensor(O.cast_and_scale(ref).astype(np.float32), device="cuda"), oth)
+    g = adj.backward()
+    ga = np.array([g[k] for k in A.PARAMS])
+    # float64 autograd of the restatement
+    sizes = O.compute_pyramid_sizes(n, n, nl, 2.0)
+    masks = [torch.as_tensor(m > 0) for m in O.create_mask_pyramid(mask, sizes)]
+    trs = [torch.as_tensor(t) for t in O.create_vector_field_pyramid(tvf, sizes)]
+    ow = torch.as_tensor(O.warping_image(O.cast_and_scale(oth), cvf))
+    th = [torch.tensor(theta[k], dtype=torch.float64, requires_grad=True) for k in A.PARAMS]
+    flow = R.solve_stereo(torch.as_tensor(O.cast_and_scale(ref)), ow, masks, trs, sizes, *th, nw, ni)
+    R.depth_loss(flow, cam, torch.as_tensor(pr["t"].reshape(3)), masks[0], torch.as_tensor(zg)).backward()
+    g64 = np.array([x.grad.item() for x in th])
+
+    def oloss(tt):
+        p = O.OracleParams(n, n, nLevel=nl, fScale=2.0, nWarpIters=nw, nSolverIters=ni, **tt)
+        fl = O.solve_stereo(ref, oth, mask, tvf, cvf, p)
+        z = O.triangulate_flow(fl, cam, pr["t"].reshape(3), mask)[..., 2]
+        m = (mask > 0) & (z > 0) & np.isfinite(z) & (zg > 0)
+        return float(np.mean(np.abs(np.log(z[m]) - np.log(zg[m]))))
+
+    h = 1e-5
+    fd = []
+    for k in A.PARAMS:
+        tp, tm = dict(theta), dict(theta)
+        tp[k] *= 1 + h
+        tm[k] *= 1 - h
+        fd.append((oloss(tp) - oloss(tm)) / (2 * h * theta[k]))
+    fd = np.array(fd)
+    for k, a_, c_, d_ in zip(A.PARAMS, ga, g64, fd):
+        print(f"{k}: adjoint {a_:.8g} float64 autograd {c_:.8g} central difference {d_:.8g}")
+    r_fd = np.linalg.norm(fd - g64) / np.linalg.norm(g64)
+    r_adj = np.linalg.norm(ga - fd) / np.linalg.norm(fd)
+    print(f"|fd - autograd| / |autograd| = {r_fd:.2e}; |adjoint - fd| / |fd| = {r_adj:.2e}")
+    assert r_fd <= 1e-3
+    assert r_adj <= 1e-3

@@ -64,7 +64,7 @@ def _oracle_depth_loss(pr, tvf, cvf, theta, n_warp, n_iter):
     prm = O.OracleParams(n, n, nLevel=2, fScale=2.0, nWarpIters=n_warp, nSolverIters=n_iter, **theta)
     flow = O.solve_stereo(pr["img_ref"], pr["img_other"], pr["mask"], tvf, cvf, prm)
     z = O.triangulate_flow(flow, pr["cam"], pr["t"].reshape(3), pr["mask"])[..., 2]
-    m = (pr["mask"] > 0) & (z > 0) & np.isfinite(z)
+    m = (pr["mask"] > 0) & (z > 0) & np.isfinite(z) & (pr["z_gt"] > 0)
     return float(np.mean(np.abs(np.log(z[m]) - np.log(pr["z_gt"][m]))))
 
 
