@@ -214,8 +214,75 @@ def measured_traffic(per_level):
     return round(tot_b / tot_n)
 
 
-def run_batch(a):
-    """64 pairs of uneven size (16 each of 512^2, 800^2, 1024^2, 1280x960) through VFSBatch."""
+def batched_kernel_roofline(batch):
+    """Average duration and algorithmic bandwidth of the batched blocked primal-dual kernel
+    over one batch solve's launches.  Per level, one warp of the batch's blocked launches
+    (vfs_pd_blocks_batch: every pair's tiles in each launch) is recorded into a HIP graph
+    and replayed between HIP events on the launch stream, on the batch's own level data.
+    Bytes per launch: the sum over pairs of the single-pair formula of
+    blocked_kernel_roofline (73 B per pixel of each extended tile, 41 B in a warp's first
+    launch, 40 B written per tile pixel)."""
+    import ctypes
+    import torch
+    import vfs_hip as H
+    p = batch.param
+    n_iter, n_warp = int(p.nSolverIters), int(p.nWarpIters)
+    prm = H.make_params(p, batch.vfs[0].median_filter_kernel_size)
+    taus, sigmas, mus = H.pd_schedule(n_iter, p.timeStepLambda)
+    sched = [x.ctypes.data_as(ctypes.c_void_p) for x in (taus, sigmas, mus)]
+    n = len(batch.vfs)
+    s = H.stream()
+    total_ms, total_launches, total_bytes = 0.0, 0, 0
+    per_level = []
+    for lvl, (tx, ty, k, nt) in enumerate(batch._tiles()):
+        levels, wss = batch._structs[lvl]
+        nblk = -(-n_iter // k)
+        ext = tile_px = 0
+        for v in batch.vfs:
+            hh, ww = v.sizes[lvl]
+            tile_px += hh * ww
+            for y0 in range(0, hh, ty):
+                ey = min(hh, y0 + ty + k) - max(0, y0 - k)
+                for x0 in range(0, ww, tx):
+                    ext += ey * (min(ww, x0 + tx + k) - max(0, x0 - k))
+        warp_bytes = (41 * ext + 40 * tile_px) + (nblk - 1) * (73 * ext + 40 * tile_px)
+        args = (H.ptr(batch._ws[lvl]), ctypes.cast(levels, ctypes.c_void_p), ctypes.cast(wss, ctypes.c_void_p), n,
+                ctypes.byref(prm), *sched, n_iter, tx, ty, k, nt)
+        cs = ctypes.c_void_p()
+        H.call("vfs_capture_begin", ctypes.byref(cs))
+        try:
+            H.call("vfs_pd_blocks_batch", cs, *args)
+        finally:
+            plan = ctypes.c_void_p()
+            H.call("vfs_capture_end", cs, ctypes.byref(plan))
+        try:
+            H.call("vfs_plan_launch", plan, s)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = max(1, min(n_warp, 5))
+            e0.record()
+            for _ in range(reps):
+                H.call("vfs_plan_launch", plan, s)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+        finally:
+            H.lib().vfs_plan_destroy(plan)
+        per_level.append(dict(level=lvl, pixels=tile_px, tile=[tx, ty, k, nt], launches=nblk * n_warp,
+                              avg_us=round(1e3 * ms / nblk, 3), gbs=round(warp_bytes / (ms * 1e-3) / 1e9, 1)))
+        total_ms += ms * n_warp
+        total_launches += nblk * n_warp
+        total_bytes += warp_bytes * n_warp
+    avg_s = total_ms * 1e-3 / total_launches
+    bytes_per_launch = total_bytes / total_launches
+    return dict(avg_us=avg_s * 1e6, bytes_per_launch=bytes_per_launch,
+                achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
+
+
+def batch_leg(n_pairs=64, steps=1, warmup=1, compare=True, roofline=True, cpu_baseline=True):
+    """64 pairs of uneven size (16 each of 512^2, 800^2, 1024^2, 1280x960) through VFSBatch:
+    pairs/s of the batched solve (+ triangulation), the per-kernel path one pair at a time
+    beside it, the batched kernel's roofline and the depth error against the ground truth."""
     import torch
     import synthetic as S
     import vfs as V
@@ -224,19 +291,20 @@ def run_batch(a):
     import vector_field as VF
     import equidistant_cam as E
     spec = S.BATCH_64
-    log(f"[bench] batch: generating {a.pairs} pairs on the GPU")
-    pairs = S.batch_pairs(spec, device="cuda", n=a.pairs)
+    log(f"[bench] batch: generating {n_pairs} pairs on the GPU")
+    pairs = S.batch_pairs(spec, device="cuda", n=n_pairs)
     sizes = [p["img_ref"].shape for p in pairs]
     prm = P.vfs_params()
     S.apply_workload_params(prm, spec)
     batch = B.VFSBatch(prm, sizes)
-    cams = []
+    cams, fields = [], []
     for i, p in enumerate(pairs):
         cam = E.EquidistantCam(p["cam"])
         cam.setup_module()
         cams.append(cam)
         tvf, cvf = VF.get_vector_fields(VF.get_grid_zero(sizes[i]), cam, cam, p["R"], p["t"], 1, VF.unit_normalizer)
-        batch.setup_pair(i, p["mask"], S.sanitize_field(tvf), cvf)
+        fields.append((S.sanitize_field(tvf), cvf))
+        batch.setup_pair(i, p["mask"], *fields[-1])
     refs = [torch.as_tensor(p["img_ref"], device="cuda") for p in pairs]
     others = [torch.as_tensor(p["img_other"], device="cuda") for p in pairs]
     ts = [p["t"] for p in pairs]
@@ -246,22 +314,27 @@ def run_batch(a):
         return batch.triangulate(cams, ts)
 
     log("[bench] batch: warmup (records the graph)")
-    for _ in range(max(1, a.warmup)):
+    for _ in range(max(1, warmup)):
         step()
     torch.cuda.synchronize()
-    log(f"[bench] batch: timing {a.steps} steps")
+    log(f"[bench] batch: timing {steps} steps")
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
+    for _ in range(steps):
+        xyz = step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    errs = []
+    for x, p in zip(xyz, pairs):
+        z = x[..., 2].cpu().numpy()
+        m = (p["mask"] > 0) & np.isfinite(z) & (p["z_gt"] > 1e-3)
+        errs.append(float(np.median(np.abs(z[m] - p["z_gt"][m]) / p["z_gt"][m])))
     px = sum(sum(h * w for h, w in v.sizes) for v in batch.vfs)
     px_iter = px * spec["nWarpIters"] * spec["nSolverIters"]
     bytes_batch = sum(sum(h * w * (spec["nWarpIters"] * (spec["nSolverIters"] * B_ITER + B_WARP) + B_LEVEL)
                           for h, w in v.sizes) for v in batch.vfs)
-    value = len(pairs) * a.steps / dt
+    value = len(pairs) * steps / dt
     plain = None
-    if not a.no_compare:
+    if compare:
         # the per-kernel path, one pair at a time: one pair of each size class timed, scaled by its count
         per_size = {}
         for i, s in enumerate(sizes):
@@ -282,22 +355,55 @@ def run_batch(a):
                 v.triangulate_flow(cams[i], ts[i])
             torch.cuda.synchronize()
             per_size[s] = (time.perf_counter() - t1) / 2
+            v.release_plan()
             log(f"[bench] per-kernel path {s}: {1e3 * per_size[s]:.1f} ms/pair")
         tot = sum(per_size[s] for s in sizes)
         plain = dict(value=round(len(pairs) / tot, 3), ms_per_batch=round(1e3 * tot, 1),
-                     speedup_of_reported=round(tot / (dt / a.steps), 2))
+                     speedup_of_reported=round(tot / (dt / steps), 2))
+    roof = None
+    if roofline:
+        log("[bench] batch: batched-kernel roofline pass (HIP events)")
+        r = batched_kernel_roofline(batch)
+        roof = dict(bound="hbm", kernel="k_pd_block_batched", achieved=round(r["achieved_gbs"], 1),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
+                    per_level=r["per_level"])
+    cpu = None
+    if cpu_baseline:
+        log("[bench] batch: CPU oracle baseline (one pair per size, 1 warp per level)")
+        by_size = {}
+        for i, p in enumerate(pairs):
+            if sizes[i] not in by_size:
+                by_size[sizes[i]] = [p, fields[i][0], np.asarray(fields[i][1], np.float32), 0]
+            by_size[sizes[i]][3] += 1
+        cpu = cpu_batch_baseline({k: tuple(v) for k, v in by_size.items()}, spec)
+    batch.release_plan()
+    return dict(value=round(value, 3), unit="depth maps/s", pairs=len(pairs), steps=steps,
+                ms_per_batch=round(1e3 * dt / steps, 3), sizes=sorted({f"{w}x{h}" for h, w in sizes}),
+                levels=spec["nLevel"], warps_per_level=spec["nWarpIters"], pd_iters_per_warp=spec["nSolverIters"],
+                pd_tiles=[list(t) for t in batch._tiles()], plain_per_kernel=plain,
+                mpx_iter_per_s=round(px_iter * steps / dt / 1e6, 1),
+                solver_effective_gbs=round(bytes_batch * steps / dt / 1e9, 1),
+                median_rel_depth_err_vs_gt=round(float(np.median(errs)), 4),
+                max_rel_depth_err_vs_gt=round(float(np.max(errs)), 4), roofline=roof, cpu_baseline=cpu)
+
+
+def run_batch(a):
+    """The batched workload alone (--workload batched_64)."""
+    r = batch_leg(a.pairs, a.steps, a.warmup, compare=not a.no_compare, roofline=not a.no_roofline,
+                  cpu_baseline=not a.no_cpu_baseline)
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "depth maps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True,
+        "metric": METRIC, "value": r["value"], "unit": "depth maps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": r["ms_per_batch"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batched_64", "pairs": len(pairs),
-                   "sizes": sorted({f"{w}x{h}" for h, w in sizes}), "levels": spec["nLevel"],
-                   "warps_per_level": spec["nWarpIters"], "pd_iters_per_warp": spec["nSolverIters"],
-                   "solver": "batched blocked (one graph per batch)", "pd_tiles": [list(t) for t in batch._tiles()]},
-        "plain_per_kernel": plain,
-        "mpx_iter_per_s": round(px_iter * a.steps / dt / 1e6, 1),
-        "solver_effective_gbs": round(bytes_batch * a.steps / dt / 1e9, 1),
-        "roofline": None, "cpu_baseline": None,
+        "config": {"workload": "batched_64", "pairs": r["pairs"], "sizes": r["sizes"], "levels": r["levels"],
+                   "warps_per_level": r["warps_per_level"], "pd_iters_per_warp": r["pd_iters_per_warp"],
+                   "solver": "batched blocked (one graph per batch)", "pd_tiles": r["pd_tiles"]},
+        "plain_per_kernel": r["plain_per_kernel"],
+        "mpx_iter_per_s": r["mpx_iter_per_s"],
+        "solver_effective_gbs": r["solver_effective_gbs"],
+        "median_rel_depth_err_vs_gt": r["median_rel_depth_err_vs_gt"],
+        "roofline": r["roofline"], "cpu_baseline": r["cpu_baseline"],
     }
     print(json.dumps(out), flush=True)
 
@@ -422,21 +528,7 @@ def run_probe(a):
     print(json.dumps({"probe": a.workload, "copy_floats": n, "sizes": sizes}), flush=True)
 
 
-def cpu_oracle_baseline(ctx, name):
-    """float64 NumPy oracle (1 thread) on the full pyramid with 1 of nWarpIters warps per level."""
-    from oracle import vfs_oracle as O
-    import synthetic as S
-    w = S.WORKLOADS[name]
-    pair = ctx["pair"]
-    Hh, Ww = pair["img_ref"].shape
-    oprm = O.OracleParams(Hh, Ww, nLevel=w["nLevel"], fScale=w.get("fScale", 2.0), nWarpIters=1,
-                          nSolverIters=w["nSolverIters"],
-                          Lambda=w["Lambda"], alpha0=w["alpha0"], alpha1=w["alpha1"], beta=w["beta"],
-                          gamma=w["gamma"])
-    t0 = time.perf_counter()
-    O.solve_stereo(pair["img_ref"], pair["img_other"], pair["mask"], ctx["tvf"], ctx["cvf"], oprm)
-    dt = time.perf_counter() - t0
-    full = dt * w["nWarpIters"]
+def _host_info():
     try:
         aff = len(os.sched_getaffinity(0))
     except Exception:
@@ -450,11 +542,73 @@ def cpu_oracle_baseline(ctx, name):
                     break
     except OSError:
         pass
-    return dict(value=1.0 / full, unit="depth maps/s", cores=1, kind="port",
-                sample=(f"float64 NumPy oracle, 1 thread, {Hh}x{Ww} {w['nLevel']}-level pyramid with 1 of "
-                        f"{w['nWarpIters']} warps per level ({w['nSolverIters']} PD iterations): {dt:.1f} s, "
-                        f"extrapolated x{w['nWarpIters']} to one map"),
-                host=dict(cpu_count=os.cpu_count(), affinity=aff, cpu_model=cpu))
+    return dict(cpu_count=os.cpu_count(), affinity=aff, cpu_model=cpu)
+
+
+def _oracle_params(name_or_spec, hh, ww, n_warp=None):
+    from oracle import vfs_oracle as O
+    import synthetic as S
+    w = S.WORKLOADS[name_or_spec] if isinstance(name_or_spec, str) else name_or_spec
+    prm = O.OracleParams(hh, ww)
+    S.apply_workload_params(prm, w)
+    if n_warp is not None:
+        prm.nWarpIters = n_warp
+    return prm
+
+
+def cpu_oracle_baseline(ctx, name):
+    """The float64 C restatement of the oracle (oracle/vfs_oracle.c) on the host: one whole
+    solve of the same pair and schedule plus triangulation, on every core of the process's
+    affinity (OpenMP over image rows; the flow does not depend on the thread count), and a
+    bounded single-thread sample (1 warp per level, extrapolated)."""
+    from oracle import vfs_oracle as O
+    from oracle import vfs_oracle_c as C
+    import synthetic as S
+    w = S.WORKLOADS[name]
+    pair = ctx["pair"]
+    hh, ww = pair["img_ref"].shape
+    host = _host_info()
+    nt = int(host["affinity"] or 1)
+    args = (pair["img_ref"], pair["img_other"], pair["mask"], ctx["tvf"], np.asarray(ctx["cvf"], np.float32))
+    t0 = time.perf_counter()
+    flow = C.solve_stereo(*args, _oracle_params(name, hh, ww), threads=nt)
+    O.triangulate_flow(flow, pair["cam"], pair["t"].reshape(3), pair["mask"])
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    C.solve_stereo(*args, _oracle_params(name, hh, ww, n_warp=1), threads=1)
+    dt1 = (time.perf_counter() - t1) * w["nWarpIters"]
+    return dict(value=1.0 / dt, unit="depth maps/s", cores=nt, kind="port",
+                sample=(f"float64 C restatement of the oracle (oracle/vfs_oracle.c), one whole {hh}x{ww} solve "
+                        f"({w['nLevel']} levels x {w['nWarpIters']} warps x {w['nSolverIters']} iterations) and "
+                        f"triangulation on {nt} threads: {dt:.1f} s"),
+                single_thread=dict(value=1.0 / dt1, unit="depth maps/s", cores=1,
+                                   sample=(f"1 thread, 1 of {w['nWarpIters']} warps per level: {dt1 / w['nWarpIters']:.1f} s, "
+                                           f"extrapolated x{w['nWarpIters']}")),
+                host=host)
+
+
+def cpu_batch_baseline(pairs_by_size, spec):
+    """The C oracle on the batched workload: one pair of each size class with 1 warp per
+    level on every core, extrapolated to the whole schedule and to the 16 pairs of each
+    class (labelled extrapolated)."""
+    from oracle import vfs_oracle as O
+    from oracle import vfs_oracle_c as C
+    host = _host_info()
+    nt = int(host["affinity"] or 1)
+    total, parts = 0.0, []
+    for (hh, ww), (p, tvf, cvf, count) in pairs_by_size.items():
+        t0 = time.perf_counter()
+        flow = C.solve_stereo(p["img_ref"], p["img_other"], p["mask"], tvf, cvf,
+                              _oracle_params(spec, hh, ww, n_warp=1), threads=nt)
+        O.triangulate_flow(flow, p["cam"], np.asarray(p["t"]).reshape(3), p["mask"])
+        dt = time.perf_counter() - t0
+        total += dt * spec["nWarpIters"] * count
+        parts.append(f"{ww}x{hh} {dt:.2f} s")
+    n = sum(c for (_, _, _, c) in pairs_by_size.values())
+    return dict(value=n / total, unit="depth maps/s", cores=nt, kind="port",
+                sample=(f"float64 C restatement of the oracle on {nt} threads, one pair of each size with 1 of "
+                        f"{spec['nWarpIters']} warps per level ({', '.join(parts)}), extrapolated x{spec['nWarpIters']} "
+                        f"and to {n} pairs"), host=host)
 
 
 def main():
@@ -471,6 +625,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-pair batched leg of the default run")
     ap.add_argument("--save-flow", default=None, help="write the last step's flow (H, W, 2) to this .npy file")
     ap.add_argument("--probe", action="store_true",
                     help="only the launches of the hardware-counter runs (copy kernel + blocked kernel per level)")
@@ -563,8 +718,15 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        log("[bench] CPU oracle baseline (bounded sample)")
+        log("[bench] CPU oracle baseline")
         cpu = cpu_oracle_baseline(ctx, a.workload)
+
+    # the batched workload beside the headline: one timed batch of the 64 pairs
+    batched = None
+    if a.workload == "plain_full_800" and not a.no_batch:
+        v.release_plan()
+        batched = batch_leg(64, steps=1, warmup=1, compare=not a.no_compare, roofline=not a.no_roofline,
+                            cpu_baseline=not a.no_cpu_baseline)
 
     out = {
         "metric": METRIC,
@@ -591,6 +753,7 @@ def main():
         "median_rel_depth_err_vs_gt": round(depth_err, 4),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_64": batched,
     }
     print(json.dumps(out), flush=True)
 

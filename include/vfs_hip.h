@@ -294,6 +294,15 @@ int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const v
                                   int n_iter, int n_warp, int tile_x, int tile_y, int k,
                                   int threads);
 
+/* The blocked primal-dual launches alone of one warp of the batch's level
+ * (ceil(n_iter / k) batched launches over the tiles of all pairs, from the warp's
+ * first launch, state sets alternating as in vfs_solve_level_blocked_batch), for
+ * timing the batched kernel; it overwrites the pairs' primal-dual state.
+ * d_workspace as for vfs_solve_level_blocked_batch. */
+int vfs_pd_blocks_batch(void* stream, const void* d_workspace, const vfs_level* lv, const vfs_pd_ws* ws,
+                        int n_pairs, const vfs_params* prm, const float* taus, const float* sigmas,
+                        const float* mus, int n_iter, int tile_x, int tile_y, int k, int threads);
+
 /* ---- adjoint (reverse mode) of the solve (no reference counterpart) -------------- */
 
 /* Buffers of one reverse primal-dual iteration at one pyramid level.  The forward

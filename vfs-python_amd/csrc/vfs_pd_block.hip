@@ -876,6 +876,38 @@ int vfs_batch_prepare_level(void* d_workspace, size_t workspace_bytes, const vfs
     return e == hipSuccess ? 0 : static_cast<int>(e);
 }
 
+int vfs_pd_blocks_batch(void* stream, const void* d_workspace, const vfs_level* lv, const vfs_pd_ws* ws, int n_pairs,
+                        const vfs_params* prm, const float* taus, const float* sigmas, const float* mus, int n_iter,
+                        int tile_x, int tile_y, int k, int threads)
+{
+    if (d_workspace == nullptr || lv == nullptr || ws == nullptr || prm == nullptr || n_pairs <= 0) return VFS_EINVAL;
+    if (n_iter < 1 || taus == nullptr || sigmas == nullptr || mus == nullptr) return VFS_EINVAL;
+    const Variant* v = find_variant(tile_x, tile_y, k, threads);
+    if (v == nullptr) return VFS_EINVAL;
+    hipStream_t s = S(stream);
+    const BatchLayout lay = batch_layout(n_pairs);
+    const PDLevel* pd = reinterpret_cast<const PDLevel*>(static_cast<const unsigned char*>(d_workspace) + lay.pd);
+    const int* pd_start = reinterpret_cast<const int*>(static_cast<const unsigned char*>(d_workspace) + lay.pd_start);
+    int pd_tiles = 0;
+    for (int i = 0; i < n_pairs; ++i) {
+        VFS_TRY(check_pair(&lv[i], &ws[i]));
+        pd_tiles += ((lv[i].W + tile_x - 1) / tile_x) * ((lv[i].H + tile_y - 1) / tile_y);
+    }
+    const PDConsts cst = make_consts(prm);
+    int cur = 0, done = 0;
+    bool first = true;
+    while (done < n_iter) {
+        const int kc = (n_iter - done) < k ? (n_iter - done) : k;
+        const PDSched sc = make_sched(prm, taus + done, sigmas + done, mus + done, kc);
+        hipLaunchKernelGGL(v->batched, dim3(pd_tiles), dim3(v->nt), 0, s, pd + ((first ? 0 : 2) + cur) * n_pairs,
+                           pd_start, n_pairs, kc, sc, cst);
+        cur = 1 - cur;
+        done += kc;
+        first = false;
+    }
+    VFS_LAUNCH_STATUS();
+}
+
 int vfs_solve_level_blocked_batch(void* stream, const void* d_workspace, const vfs_level* lv, const vfs_pd_ws* ws,
                                   int n_pairs, const vfs_params* prm, const float* taus, const float* sigmas,
                                   const float* mus, int n_iter, int n_warp, int tile_x, int tile_y, int k,

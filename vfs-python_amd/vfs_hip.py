@@ -108,6 +108,8 @@ _SIGS = {
     "vfs_batch_prepare_level": [c_p, ctypes.c_size_t, c_p, c_p, c_i, c_i, c_i, c_i, c_i],
     "vfs_solve_level_blocked_batch": [c_p, c_p, c_p, c_p, c_i, ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i,
                                       c_i, c_i, c_i, c_i],
+    "vfs_pd_blocks_batch": [c_p, c_p, c_p, c_p, c_i, ctypes.POINTER(VfsParams), c_p, c_p, c_p, c_i, c_i, c_i, c_i,
+                            c_i],
     "vfs_adj_pd_iteration": [c_p, ctypes.POINTER(VfsAdjPd), ctypes.POINTER(VfsParams), c_f, c_f, c_f, c_i],
     "vfs_adj_pd_finish": [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i, c_i],
     "vfs_adj_clamp": [c_p] * 10 + [c_i, c_i, c_f, c_i],
