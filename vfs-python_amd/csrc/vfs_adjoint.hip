@@ -303,31 +303,132 @@ __device__ __forceinline__ float tap_share(int r, int H, int W, float x, float y
 // g(Iu) / 24, and its centre by -g(Iz) (reference) and +g(Iz) (warped).  |k e| <= 2,
 // so a tap's corners lie within [s - 2, s + 3] per axis and, after the mirror,
 // every sender of texel r lies within r +- 3.  Gather: gref += T - g(Iz),
-// gow += T + g(Iz), with T the taps' shares at r summed over the 7 x 7 window in
-// row-major order.
+// gow += T + g(Iz), with T the taps' shares at r.
+//
+// A block stages (giu, e) of its receivers' senders (the block extended by 3 on
+// every side) in LDS.  Where e varies little over a receiver's 7 x 7 senders
+// (every component within +-0.2 of the window's centre value) and the receiver is
+// at least 3 pixels from the image border (so no mirrored corner can fall on it),
+// only the senders whose tap k can reach it are visited: a corner is r when the tap
+// lies in [r - 1, r + 1), so s lies in [r - 1 - k e(s), r + 1 - k e(s)), at most
+// three integers per axis for |k| <= 2 and |e(s) - centre| <= 0.2 (checked exactly
+// per candidate).  Elsewhere (border pixels, directions that turn fast) every
+// sender of the window is visited.  Either way the shares are summed in a fixed
+// order, so the result is deterministic.
+namespace {
+constexpr int DR = 3, DTX = VFS_BLOCK_X, DTY = VFS_BLOCK_Y, DLX = DTX + 2 * DR, DLY = DTY + 2 * DR;
+constexpr float kDirSpread = 0.2f;
+
+// the k-th tap of sender (x, y) with unit direction (ex, ey), as derivative_pixel forms it
+__device__ __forceinline__ void deriv_tap(int k, float x, float y, float ex, float ey, float& tx, float& ty, float& wt)
+{
+    if (k == -2) {
+        tx = fmaf(-2.0f, ex, x);
+        ty = fmaf(-2.0f, ey, y);
+        wt = 1.0f;
+    } else if (k == -1) {
+        tx = x - ex;
+        ty = y - ey;
+        wt = -8.0f;
+    } else if (k == 1) {
+        tx = x + ex;
+        ty = y + ey;
+        wt = 8.0f;
+    } else {
+        tx = fmaf(2.0f, ex, x);
+        ty = fmaf(2.0f, ey, y);
+        wt = -1.0f;
+    }
+}
+}  // namespace
+
 __global__ void __launch_bounds__(256) k_adj_derivatives(float* __restrict__ gref, float* __restrict__ gow,
                                                          const float* __restrict__ gderiv,
                                                          const float2* __restrict__ tvf, int H, int W)
 {
-    VFS_ADJ_PIXEL(H, W)
-    float acc = 0.0f;
-    for (int dy = -3; dy <= 3; ++dy) {
-        const int sy = Y + dy;
-        if (sy < 0 || sy >= H) continue;
-        for (int dx = -3; dx <= 3; ++dx) {
-            const int sx = X + dx;
-            if (sx < 0 || sx >= W) continue;
+    __shared__ float s_g[DLY][DLX], s_ex[DLY][DLX], s_ey[DLY][DLX];
+    const int X0 = blockIdx.x * DTX, Y0 = blockIdx.y * DTY;
+    const int tid = threadIdx.y * DTX + threadIdx.x;
+    for (int t = tid; t < DLX * DLY; t += DTX * DTY) {
+        const int lx = t % DLX, ly = t / DLX, sx = X0 - DR + lx, sy = Y0 - DR + ly;
+        float g = 0.0f, ex = 0.0f, ey = 0.0f;
+        if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
             const int s = sy * W + sx;
-            const float giu = gderiv[2 * s] / 24.0f;
-            if (giu == 0.0f) continue;
-            const float2 vec = tvf[s];
-            const float nrm = hypotf(vec.x, vec.y);
-            const float ex = vec.x / nrm, ey = vec.y / nrm;
-            const float x = static_cast<float>(sx), y = static_cast<float>(sy);
-            acc += tap_share(idx, H, W, fmaf(-2.0f, ex, x), fmaf(-2.0f, ey, y), giu * 1.0f);
-            acc += tap_share(idx, H, W, x - ex, y - ey, giu * -8.0f);
-            acc += tap_share(idx, H, W, x + ex, y + ey, giu * 8.0f);
-            acc += tap_share(idx, H, W, fmaf(2.0f, ex, x), fmaf(2.0f, ey, y), giu * -1.0f);
+            g = gderiv[2 * s] / 24.0f;
+            if (g != 0.0f) {
+                const float2 vec = tvf[s];
+                const float nrm = hypotf(vec.x, vec.y);
+                ex = vec.x / nrm;
+                ey = vec.y / nrm;
+            }
+        }
+        s_g[ly][lx] = g;
+        s_ex[ly][lx] = ex;
+        s_ey[ly][lx] = ey;
+    }
+    __syncthreads();
+    const int X = X0 + threadIdx.x, Y = Y0 + threadIdx.y;
+    if (X >= W || Y >= H) return;
+    const int idx = Y * W + X;
+    const int lx0 = threadIdx.x + DR, ly0 = threadIdx.y + DR;
+    // spread of the directions over the active senders of the 7 x 7 window
+    float mnx = 2.0f, mxx = -2.0f, mny = 2.0f, mxy = -2.0f;
+    bool finite = true;
+    for (int dy = -DR; dy <= DR; ++dy)
+        for (int dx = -DR; dx <= DR; ++dx) {
+            if (s_g[ly0 + dy][lx0 + dx] == 0.0f) continue;
+            const float ex = s_ex[ly0 + dy][lx0 + dx], ey = s_ey[ly0 + dy][lx0 + dx];
+            finite = finite && (ex == ex) && (ey == ey);
+            mnx = fminf(mnx, ex);
+            mxx = fmaxf(mxx, ex);
+            mny = fminf(mny, ey);
+            mxy = fmaxf(mxy, ey);
+        }
+    const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny);
+    const bool interior = X >= DR && X < W - DR && Y >= DR && Y < H - DR;
+    float acc = 0.0f;
+    if (interior && finite && hx <= kDirSpread && hy <= kDirSpread) {
+        const float cx = 0.5f * (mxx + mnx), cy = 0.5f * (mxy + mny);
+        for (int k = -2; k <= 2; ++k) {
+            if (k == 0) continue;
+            const float kk = static_cast<float>(k), ak = fabsf(kk);
+            // s in [r - 1 - k c - |k| h, r + 1 - k c + |k| h), widened by 0.01 for rounding
+            int xlo = static_cast<int>(ceilf(X - 1.0f - kk * cx - ak * hx - 0.01f));
+            int xhi = static_cast<int>(ceilf(X + 1.0f - kk * cx + ak * hx + 0.01f)) - 1;
+            int ylo = static_cast<int>(ceilf(Y - 1.0f - kk * cy - ak * hy - 0.01f));
+            int yhi = static_cast<int>(ceilf(Y + 1.0f - kk * cy + ak * hy + 0.01f)) - 1;
+            xlo = xlo < X - DR ? X - DR : xlo;
+            xhi = xhi > X + DR ? X + DR : xhi;
+            ylo = ylo < Y - DR ? Y - DR : ylo;
+            yhi = yhi > Y + DR ? Y + DR : yhi;
+            for (int sy = ylo; sy <= yhi; ++sy)
+                for (int sx = xlo; sx <= xhi; ++sx) {
+                    const int lx = sx - X0 + DR, ly = sy - Y0 + DR;
+                    const float g = s_g[ly][lx];
+                    if (g == 0.0f) continue;
+                    float tx, ty, wt;
+                    deriv_tap(k, static_cast<float>(sx), static_cast<float>(sy), s_ex[ly][lx], s_ey[ly][lx], tx, ty,
+                              wt);
+                    acc += tap_share(idx, H, W, tx, ty, g * wt);
+                }
+        }
+    } else {
+        for (int dy = -DR; dy <= DR; ++dy) {
+            const int sy = Y + dy;
+            if (sy < 0 || sy >= H) continue;
+            for (int dx = -DR; dx <= DR; ++dx) {
+                const int sx = X + dx;
+                if (sx < 0 || sx >= W) continue;
+                const float g = s_g[ly0 + dy][lx0 + dx];
+                if (g == 0.0f) continue;
+                const float ex = s_ex[ly0 + dy][lx0 + dx], ey = s_ey[ly0 + dy][lx0 + dx];
+                for (int k = -2; k <= 2; ++k) {
+                    if (k == 0) continue;
+                    float tx, ty, wt;
+                    deriv_tap(k, static_cast<float>(sx), static_cast<float>(sy), ex, ey, tx, ty, wt);
+                    acc += tap_share(idx, H, W, tx, ty, g * wt);
+                }
+            }
         }
     }
     const float giz = gderiv[2 * idx + 1];
