@@ -94,7 +94,7 @@ def _variants():
         return []
 
 
-@pytest.mark.parametrize("variant", _variants(), ids=lambda v: f"{v[0]}x{v[1]}k{v[2]}")
+@pytest.mark.parametrize("variant", _variants(), ids=lambda v: f"{v[0]}x{v[1]}k{v[2]}t{v[3]}")
 @pytest.mark.parametrize("shape,border,first", [((70, 53), False, 0), ((97, 131), True, 1), ((8, 9), True, 0)])
 def test_blocked_iterations_equal_per_kernel_iterations(T, variant, shape, border, first):
     import vfs_hip as H

@@ -112,14 +112,87 @@ __device__ __forceinline__ void pd_tape_write(const PDLevel& L, int k, int ox, i
     }
 }
 
+// One pixel's inputs of a tile launch as they come from memory (the forward state,
+// abc, eta, deriv) and its image-position flags.
+struct PDRaw {
+    float ub, vbx, vby, u, px, py, qx, qy, qz, qw, a, b, c, e0, e1, e2, iu, iz;
+    unsigned f;  // F_IMG, F_ACT (mask), F_HE .. F_HN
+};
+
+// Global loads of the tile (tx, ty) extended by K: every pixel of the region in the
+// image, its u_bar / v_bar (also where masked: the dual step of an inside pixel whose
+// east (south) neighbour is masked reads the west (north) one), and inside the mask
+// u, p, q and the constants.  Zeros elsewhere, and for v_bar, p, q at a warp's first launch.
 template <int TX, int TY, int K, int NT>
-__device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, int k_count, const PDSched& sc,
-                                              const PDConsts& cst, float* __restrict__ sA, float* __restrict__ sB)
+__device__ __forceinline__ void pd_load_raw(const PDLevel& L, int tx, int ty,
+                                            PDRaw (&r)[((TX + 2 * K) * (TY + 2 * K) + NT - 1) / NT])
 {
     constexpr int EX = TX + 2 * K, EY = TY + 2 * K, E = EX * EY;
     constexpr int S = (E + NT - 1) / NT;
     const int tid = threadIdx.x;
     const int H = L.H, W = L.W;
+    const int ox = tx * TX - K, oy = ty * TY - K;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int e = tid + NT * j;
+        const int ex = e % EX, ey = e / EX;
+        const int gx = ox + ex, gy = oy + ey;
+        PDRaw& q = r[j];
+        unsigned f = 0;
+        q.ub = q.vbx = q.vby = q.u = q.px = q.py = q.qx = q.qy = q.qz = q.qw = 0.0f;
+        q.a = q.b = q.c = q.e0 = q.e1 = q.e2 = q.iu = q.iz = 0.0f;
+        if (e < E && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int idx = gy * W + gx;
+            f |= F_IMG;
+            if (gx < W - 1 && ex < EX - 1) f |= F_HE;
+            if (gx > 0 && ex > 0) f |= F_HW;
+            if (gy < H - 1 && ey < EY - 1) f |= F_HS;
+            if (gy > 0 && ey > 0) f |= F_HN;
+            q.ub = L.ub_in[idx];
+            if (!L.first) {
+                const float2 v = reinterpret_cast<const float2*>(L.vb_in)[idx];
+                q.vbx = v.x;
+                q.vby = v.y;
+            }
+            if (L.mask[idx] != 0) {
+                f |= F_ACT;
+                q.u = L.u_in[idx];
+                if (!L.first) {
+                    const float2 p = reinterpret_cast<const float2*>(L.p_in)[idx];
+                    const float4 qq = reinterpret_cast<const float4*>(L.q_in)[idx];
+                    q.px = p.x;
+                    q.py = p.y;
+                    q.qx = qq.x;
+                    q.qy = qq.y;
+                    q.qz = qq.z;
+                    q.qw = qq.w;
+                }
+                q.a = L.abc[3 * idx + 0];
+                q.b = L.abc[3 * idx + 1];
+                q.c = L.abc[3 * idx + 2];
+                q.e0 = L.eta[3 * idx + 0];
+                q.e1 = L.eta[3 * idx + 1];
+                q.e2 = L.eta[3 * idx + 2];
+                const float2 d = reinterpret_cast<const float2*>(L.deriv)[idx];
+                q.iu = d.x;
+                q.iz = d.y;
+            }
+        }
+        q.f = f;
+    }
+}
+
+// k_count iterations on the tile (tx, ty) from its loaded inputs r, then the write-back
+// of the tile interior (and, with L.tape, the tape of its intermediate states).
+template <int TX, int TY, int K, int NT>
+__device__ __forceinline__ void pd_tile_run(const PDLevel& L, int tx, int ty, int k_count, const PDSched& sc,
+                                            const PDConsts& cst, float* __restrict__ sA, float* __restrict__ sB,
+                                            const PDRaw (&r)[((TX + 2 * K) * (TY + 2 * K) + NT - 1) / NT])
+{
+    constexpr int EX = TX + 2 * K, EY = TY + 2 * K, E = EX * EY;
+    constexpr int S = (E + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const int W = L.W;
     const int ox = tx * TX - K, oy = ty * TY - K;
 
     float u[S], ub[S], vbx[S], vby[S], px[S], py[S], qx[S], qy[S], qz[S], qw[S];
@@ -130,51 +203,29 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
 #pragma unroll
     for (int j = 0; j < S; ++j) {
         const int e = tid + NT * j;
-        const int ex = e % EX, ey = e / EX;
-        const int gx = ox + ex, gy = oy + ey;
-        unsigned f = 0;
-        u[j] = ub[j] = vbx[j] = vby[j] = px[j] = py[j] = qx[j] = qy[j] = qz[j] = qw[j] = 0.0f;
-        ca[j] = cb[j] = cc[j] = r0[j] = r1[j] = r2[j] = iu[j] = riu[j] = iz[j] = 0.0f;
-        if (e < E && gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const int idx = gy * W + gx;
-            f |= F_IMG;
-            if (gx < W - 1 && ex < EX - 1) f |= F_HE;
-            if (gx > 0 && ex > 0) f |= F_HW;
-            if (gy < H - 1 && ey < EY - 1) f |= F_HS;
-            if (gy > 0 && ey > 0) f |= F_HN;
-            // u_bar / v_bar are loaded at masked pixels too: the dual step of an
-            // inside pixel whose east (south) neighbour is masked reads the west
-            // (north) neighbour whether or not that one is masked.
-            ub[j] = L.ub_in[idx];
-            if (!L.first) {
-                const float2 v = reinterpret_cast<const float2*>(L.vb_in)[idx];
-                vbx[j] = v.x;
-                vby[j] = v.y;
-            }
-            if (L.mask[idx] != 0) {
-                f |= F_ACT;
-                u[j] = L.u_in[idx];
-                if (!L.first) {
-                    const float2 p = reinterpret_cast<const float2*>(L.p_in)[idx];
-                    const float4 q = reinterpret_cast<const float4*>(L.q_in)[idx];
-                    px[j] = p.x;
-                    py[j] = p.y;
-                    qx[j] = q.x;
-                    qy[j] = q.y;
-                    qz[j] = q.z;
-                    qw[j] = q.w;
-                }
-                ca[j] = L.abc[3 * idx + 0];
-                cb[j] = L.abc[3 * idx + 1];
-                cc[j] = L.abc[3 * idx + 2];
-                r0[j] = recip_eta(L.eta[3 * idx + 0], kEps);
-                r1[j] = recip_eta(L.eta[3 * idx + 1], kEps);
-                r2[j] = recip_eta(L.eta[3 * idx + 2], kEps);
-                const float2 d = reinterpret_cast<const float2*>(L.deriv)[idx];
-                iu[j] = d.x;
-                riu[j] = recip_grad(d.x, kEps);
-                iz[j] = d.y;
-            }
+        const PDRaw& q = r[j];
+        const unsigned f = q.f;
+        u[j] = q.u;
+        ub[j] = q.ub;
+        vbx[j] = q.vbx;
+        vby[j] = q.vby;
+        px[j] = q.px;
+        py[j] = q.py;
+        qx[j] = q.qx;
+        qy[j] = q.qy;
+        qz[j] = q.qz;
+        qw[j] = q.qw;
+        ca[j] = q.a;
+        cb[j] = q.b;
+        cc[j] = q.c;
+        r0[j] = r1[j] = r2[j] = iu[j] = riu[j] = iz[j] = 0.0f;
+        if (f & F_ACT) {
+            r0[j] = recip_eta(q.e0, kEps);
+            r1[j] = recip_eta(q.e1, kEps);
+            r2[j] = recip_eta(q.e2, kEps);
+            iu[j] = q.iu;
+            riu[j] = recip_grad(q.iu, kEps);
+            iz[j] = q.iz;
         }
         fl[j] = f;
         if (e < E) {
@@ -302,6 +353,16 @@ __device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, 
         reinterpret_cast<float2*>(L.p_out)[idx] = make_float2(px[j], py[j]);
         reinterpret_cast<float4*>(L.q_out)[idx] = make_float4(qx[j], qy[j], qz[j], qw[j]);
     }
+}
+
+
+template <int TX, int TY, int K, int NT>
+__device__ __forceinline__ void pd_block_tile(const PDLevel& L, int tx, int ty, int k_count, const PDSched& sc,
+                                              const PDConsts& cst, float* __restrict__ sA, float* __restrict__ sB)
+{
+    PDRaw r[((TX + 2 * K) * (TY + 2 * K) + NT - 1) / NT];
+    pd_load_raw<TX, TY, K, NT>(L, tx, ty, r);
+    pd_tile_run<TX, TY, K, NT>(L, tx, ty, k_count, sc, cst, sA, sB, r);
 }
 
 template <int TX, int TY, int K, int NT>
