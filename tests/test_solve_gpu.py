@@ -7,11 +7,15 @@ alpha0 = 5, alpha1 = 1, beta = 10, gamma = 0.5.
 Tolerances (fp32 GPU against the float64 oracle after the full iteration count):
   * coarsest level (64x64, after its 10 warps): median |d flow| <= 1e-6 px,
     99th percentile <= 1e-3 px;
-  * full resolution: median |d flow| <= 1e-4 px; 90th percentile <= 5e-2 px and
-    99th percentile <= 0.5 px.  The tail is the iteration's amplification of
-    rounding: the float32 NumPy restatement of the same formulas deviates from
-    the float64 oracle with the same distribution (median 3.7e-5, p90 1.2e-2,
-    p99 0.2 px), so the GPU is also held to <= 3x that restatement's p99;
+  * full resolution: median |d flow| <= 1e-4 px (the north star's figure holds as a
+    median); 90th percentile <= 5e-2 px and 99th percentile <= 0.5 px.  Measured:
+    median 3.6e-5, p90 1.4e-2, p99 0.20, max 12.1 px.  The tail is the iteration's
+    amplification of rounding, not a kernel error: this workload's lambda = 50 makes
+    the solve diverge locally (a few pixels run off by many pixels in any precision),
+    and the float32 NumPy restatement of the same formulas deviates from the float64
+    oracle with the same distribution (median 3.7e-5, p90 1.3e-2, p99 0.20, max
+    9.4 px), so no per-pixel bound like max |d flow| <= 1e-4 px is meaningful here;
+    the GPU is also held to <= 3x that restatement's p99;
   * depth: median relative depth difference <= 1e-3 over the mask;
   * the final mask (flow zero outside it) is exact.
 """
