@@ -348,7 +348,8 @@ def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt, rows):
 @pytest.mark.parametrize("shape,ckpt,rb", [((54, 70), 4, 2), ((48, 48), 9, 3), ((54, 70), 5, 4), ((61, 47), 4, 1),
                                            ((54, 70), 9, (2, 512)), ((37, 90), 4, (1, 512)), ((61, 47), 7, 5),
                                            ((54, 70), 9, 6), ((37, 90), 9, (3, 512)), ((17, 23), 4, 3),
-                                           ((17, 23), 9, (2, 512))])
+                                           ((17, 23), 9, (2, 512)), ((54, 70), 9, (3, 512, "lean")),
+                                           ((37, 90), 4, (1, 512, "lean")), ((17, 23), 9, (2, 512, "lean"))])
 def test_blocked_sweep_matches_fused_sweep(T, shape, ckpt, rb):
     """The temporally blocked reverse sweep (K units per launch, adjoint state and parameter
     shares in registers, messages in LDS) against the fused sweep, bit for bit: the

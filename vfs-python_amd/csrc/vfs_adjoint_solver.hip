@@ -644,7 +644,12 @@ __device__ __forceinline__ UnitTape load_unit_tape(const BlockRevArgs& a, int u,
     return t;
 }
 
-template <int TX, int TY, int KR, int NT>
+// LEAN: one buffer of the units' forward state in LDS instead of two (a third barrier
+// per unit orders its rewrite after the previous unit's reads) and each unit's tape
+// loaded at the unit's start instead of during the previous unit: 22 instead of 31
+// floats of LDS per extended pixel and fewer registers, so that three 512-thread
+// workgroups fit on a CU instead of two.
+template <int TX, int TY, int KR, int NT, bool LEAN = false>
 __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
 {
     constexpr int EX = TX + 2 * KR, EY = TY + 2 * KR, E = EX * EY, S = (E + NT - 1) / NT;
@@ -653,7 +658,7 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
     __shared__ float sDM[9 * E];     // dual^T messages (DualMsg um[3], vm[6])
     // the unit's forward state S_i as its neighbours read it: T p (2), u_bar (1), v_bar (2),
     // q (4); two buffers, alternating between units
-    __shared__ float sFS[2][9 * E];
+    __shared__ float sFS[LEAN ? 1 : 2][9 * E];
     __shared__ unsigned sFL[E];
     const int tid = threadIdx.x;
     const int H = a.H, W = a.W;
@@ -760,7 +765,12 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
 #pragma unroll
     for (int j = 0; j < S; ++j) nxt[j] = load_unit_tape(a, 0, fl[j], gi[j]);
     for (int u = 0; u < a.n_units; ++u) {
-        float* fsb = sFS[u & 1];
+        float* fsb = sFS[LEAN ? 0 : (u & 1)];
+        if (LEAN && u > 0) {
+#pragma unroll
+            for (int j = 0; j < S; ++j) nxt[j] = load_unit_tape(a, u, fl[j], gi[j]);
+            __syncthreads();  // the previous unit's reads of the forward state are done
+        }
         // own forward state of this unit
         float p1x[S], p1y[S], q1x[S], q1y[S], q1z[S], q1w[S], ub1[S], vb1x[S], vb1y[S], u2[S], ub2[S];
 #pragma unroll
@@ -778,7 +788,7 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             u2[j] = t.u2;
             ub2[j] = t.ub2;
         }
-        if (u + 1 < a.n_units) {
+        if (!LEAN && u + 1 < a.n_units) {
 #pragma unroll
             for (int j = 0; j < S; ++j) nxt[j] = load_unit_tape(a, u + 1, fl[j], gi[j]);
         }
@@ -979,14 +989,15 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
 }
 
 // Reverse variants (sweep modes): 20 + KR (1-6 units) with 1024 threads on a 32 x 32
-// extended tile, 30 + KR (1-3 units) with 512 threads on 32 x 16.
+// extended tile, 30 + KR (1-3 units) with 512 threads on 32 x 16, 40 + KR the LEAN
+// form of the latter.
 struct RevVariant {
     int mode, tx, ty, kr, nt;
 };
 constexpr RevVariant kRevVariants[] = {
     {21, 30, 30, 1, 1024}, {22, 28, 28, 2, 1024}, {23, 26, 26, 3, 1024}, {24, 24, 24, 4, 1024},
     {25, 22, 22, 5, 1024}, {26, 20, 20, 6, 1024}, {31, 30, 14, 1, 512},  {32, 28, 12, 2, 512},
-    {33, 26, 10, 3, 512},
+    {33, 26, 10, 3, 512},  {41, 30, 14, 1, 512},  {42, 28, 12, 2, 512},  {43, 26, 10, 3, 512},
 };
 
 const RevVariant* find_rev_variant(int mode)
@@ -996,10 +1007,10 @@ const RevVariant* find_rev_variant(int mode)
     return nullptr;
 }
 
-template <int TX, int TY, int KR, int NT>
+template <int TX, int TY, int KR, int NT, bool LEAN = false>
 int launch_adj_block_t(hipStream_t s, int tiles, const BlockRevArgs& ba)
 {
-    hipLaunchKernelGGL((k_adj_block<TX, TY, KR, NT>), dim3(tiles), dim3(NT), 0, s, ba);
+    hipLaunchKernelGGL((k_adj_block<TX, TY, KR, NT, LEAN>), dim3(tiles), dim3(NT), 0, s, ba);
     return hipGetLastError() == hipSuccess ? 0 : VFS_EINVAL;
 }
 
@@ -1015,6 +1026,9 @@ int launch_adj_block(const RevVariant& v, hipStream_t s, int tiles, const BlockR
         case 31: return launch_adj_block_t<30, 14, 1, 512>(s, tiles, ba);
         case 32: return launch_adj_block_t<28, 12, 2, 512>(s, tiles, ba);
         case 33: return launch_adj_block_t<26, 10, 3, 512>(s, tiles, ba);
+        case 41: return launch_adj_block_t<30, 14, 1, 512, true>(s, tiles, ba);
+        case 42: return launch_adj_block_t<28, 12, 2, 512, true>(s, tiles, ba);
+        case 43: return launch_adj_block_t<26, 10, 3, 512, true>(s, tiles, ba);
         default: return VFS_EINVAL;
     }
 }

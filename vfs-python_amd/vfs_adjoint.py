@@ -60,10 +60,11 @@ class VFSDepthAdjoint:
         self._auto_block = False
         # temporally blocked sweep: reverse_block = K (1-6) units per launch on a 32 x 32
         # extended tile (1024 threads; sweep 20 + K), or (K, 512) for 1-3 units on 32 x 16;
-        # a dict {level: choice} sets it per pyramid level (missing levels: fused); "auto"
-        # picks by level size (three units on the 512-thread tile at >= 400 k pixels, three
-        # on the 1024-thread tile at >= 100 k, six below: measured at the 800 x 800 gradient
-        # workload)
+        # a dict {level: choice} sets it per pyramid level (missing levels: fused); (K, 512,
+        # "lean") is the 512-thread tile with one forward-state buffer in LDS and no tape
+        # prefetch, three workgroups per CU instead of two; "auto" picks by level size (the
+        # lean three-unit 512-thread tile at >= 400 k pixels, three units on the 1024-thread
+        # tile at >= 100 k, six below: measured at the 800 x 800 gradient workload)
         self._level_mode = {}
         if reverse_block == "auto":
             reverse_block = None if (not self.fused or rows is not None) else "auto"
@@ -72,10 +73,11 @@ class VFSDepthAdjoint:
         elif reverse_block is not None:
             if not self.fused or rows is not None:
                 raise ValueError("reverse_block replaces the fused sweep's middle launches; keep fused=True, rows=None")
-            modes = {1: 21, 2: 22, 3: 23, 4: 24, 5: 25, 6: 26, (1, 512): 31, (2, 512): 32, (3, 512): 33}
+            modes = {1: 21, 2: 22, 3: 23, 4: 24, 5: 25, 6: 26, (1, 512): 31, (2, 512): 32, (3, 512): 33,
+                     (1, 512, "lean"): 41, (2, 512, "lean"): 42, (3, 512, "lean"): 43}
             per = reverse_block if isinstance(reverse_block, dict) else None
             for lvl, rb in (per.items() if per is not None else [(None, reverse_block)]):
-                key = rb if isinstance(rb, int) else tuple(rb)
+                key = rb if isinstance(rb, int) else tuple(x if isinstance(x, str) else int(x) for x in rb)
                 if key not in modes:
                     raise ValueError(f"reverse_block must be one of {list(modes)}")
                 if lvl is None:
@@ -139,7 +141,7 @@ class VFSDepthAdjoint:
             t.rec_tile_x, t.rec_tile_y, t.rec_k, t.rec_threads = rec
         if self._auto_block:
             npx = hh * ww
-            t.sweep = 33 if npx >= 400_000 else (23 if npx >= 100_000 else 26)
+            t.sweep = 43 if npx >= 400_000 else (23 if npx >= 100_000 else 26)
         else:
             t.sweep = self._level_mode.get(k, self._mode)
         t.reverse_warps = self.reverse_warps
