@@ -2,14 +2,17 @@
 
 Workload (default): one seeded synthetic pair at 800x800 (f = 800/pi, 180 deg FOV,
 baseline 0.3 m along x, rotation <= 2 deg), 5 pyramid levels (ratio 2) x 20 warps x
-50 primal-dual iterations, lambda = 50, alpha0 = 5, alpha1 = 1, beta = 10,
-gamma = 0.5, fp32.  A step is one VFS.solve_stereo (uint8 images already on the
-GPU -> flow) followed by triangulate_flow (-> depth); the timed region covers K
-steps between device synchronisations.
+50 primal-dual iterations, with the reference's default solver parameters
+(lambda = 5, alpha0 = 17, alpha1 = 1.2, beta = 9, gamma = 0.85), fp32.  A step is
+one VFS.solve_stereo (uint8 images already on the GPU -> flow) followed by
+triangulate_flow (-> depth); the timed region covers K steps between device
+synchronisations.
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
 (measured with HIP events on the launch stream), and the float64 CPU oracle
-timed on the host beside it.
+timed on the host beside it.  The default run adds two legs to the same line: the
+64-pair batched solve ("batched_64") and the depth-loss gradient on two 800x800
+pairs ("adjoint_800"), each with its own roofline and CPU baseline.
 """
 
 import argparse
@@ -133,15 +136,7 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter, levels=None):
     for (hh, ww), (tx, ty, k, nt) in zip(sizes, tiles):
         nblk = -(-n_iter // k)
 
-        def ext_px():
-            tot = 0
-            for y0 in range(0, hh, ty):
-                ey = min(hh, y0 + ty + k) - max(0, y0 - k)
-                for x0 in range(0, ww, tx):
-                    tot += ey * (min(ww, x0 + tx + k) - max(0, x0 - k))
-            return tot
-
-        ext = ext_px()
+        ext = _ext_px(hh, ww, tx, ty, k)
         warp_bytes = (41 * ext + 40 * hh * ww) + (nblk - 1) * (73 * ext + 40 * hh * ww)
         ms = autotune.time_variant(hh, ww, n_iter, (tx, ty, k, nt), reps=n_warp,
                                    level=None if levels is None else levels[len(per_level)])
@@ -408,9 +403,94 @@ def run_batch(a):
     print(json.dumps(out), flush=True)
 
 
-def run_adjoint(a):
+def _ext_px(hh, ww, tx, ty, k):
+    """Pixels of every (tx, ty) tile of an hh x ww image extended by k on each side, clipped to the image."""
+    tot = 0
+    for y0 in range(0, hh, ty):
+        ey = min(hh, y0 + ty + k) - max(0, y0 - k)
+        for x0 in range(0, ww, tx):
+            tot += ey * (min(ww, x0 + tx + k) - max(0, x0 - k))
+    return tot
+
+
+# k_adj_block algorithmic bytes: every pixel of the extended tile (clipped to the image, inside
+# the mask or not) reads the mask (1 B), the adjoint state (gu, gub 4 + 4, gvb 8, gp 8, gq 16,
+# primal^T messages 8, threshold^T message 4 = 52 B), the level inputs (abc 12, eta 12, deriv 8
+# = 32 B) and per unit the forward tape (p 8, q 16, u_bar 4, v_bar 8, u and u_bar of the
+# previous iteration 4 + 4 = 44 B); every tile pixel reads and writes its accumulators (abc,
+# eta, deriv shares 32 B, lambda / alpha0 / alpha1 shares 12 B: 44 B each way) and writes the
+# adjoint state (52 B)
+ADJ_EXT_B, ADJ_UNIT_B, ADJ_TILE_B = 85, 44, 140
+
+
+def adjoint_roofline(adj, ref0, img_other, n_launches):
+    """Average duration and algorithmic bandwidth of the temporally blocked reverse-sweep
+    kernel (k_adj_block) over the finest level's launches of one gradient, each bracketed
+    by HIP events on the stream it is launched on (VFSDepthAdjoint.backward's probe)."""
+    adj.forward(ref0, img_other)
+    probe = dict(level=0, launches=n_launches)
+    adj.backward(probe=probe)
+    tx, ty, kr, nt = probe["tiles"]
+    hh, ww = adj.sizes[0]
+    ext = _ext_px(hh, ww, tx, ty, kr)
+    nbytes = [ext * (ADJ_EXT_B + ADJ_UNIT_B * u) + ADJ_TILE_B * hh * ww for u in probe["units"]]
+    us = probe["us"]
+    avg_s = 1e-6 * sum(us) / len(us)
+    bpl = sum(nbytes) / len(nbytes)
+    return dict(avg_us=1e6 * avg_s, bytes_per_launch=bpl, achieved_gbs=bpl / avg_s / 1e9, launches=len(us),
+                tile=[tx, ty, kr, nt], size=[hh, ww], ext_px=ext, units=sorted(set(probe["units"])))
+
+
+def cpu_adjoint_baseline(pair, tvf, cvf, spec):
+    """The float64 restatement of the solve (oracle/vfs_torch_ref.py) differentiated by torch
+    autograd on the host: after an untimed warm-up run, four bounded runs of one pair at full
+    size with 1 or 2 warps per level and 2 or 6 iterations per warp are fitted (least squares)
+    to T(w, n) = A + w (B + n D), which is extrapolated to the workload's warps and
+    iterations (labelled extrapolated)."""
+    import torch
+    from oracle import vfs_oracle as O
+    from oracle import vfs_torch_ref as R
+    host = _host_info()
+    nt = int(host["affinity"] or 1)
+    prev = torch.get_num_threads()
+    torch.set_num_threads(nt)
+    try:
+        hh, ww = pair["img_ref"].shape
+        sizes = O.compute_pyramid_sizes(hh, ww, spec["nLevel"], spec.get("fScale", 2.0))
+        masks = [torch.as_tensor(m > 0) for m in O.create_mask_pyramid(pair["mask"], sizes)]
+        trs = [torch.as_tensor(t) for t in O.create_vector_field_pyramid(tvf, sizes)]
+        t = torch.as_tensor(np.asarray(pair["t"], np.float64).reshape(3))
+        z_gt = torch.as_tensor(pair["z_gt"])
+        theta = [float(spec[k]) for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma")]
+
+        def run(n_warp, n_iter):
+            t0 = time.perf_counter()
+            r0 = torch.as_tensor(O.cast_and_scale(pair["img_ref"])).requires_grad_(True)
+            ow = torch.as_tensor(O.warping_image(O.cast_and_scale(pair["img_other"]), cvf))
+            th = [torch.tensor(x, dtype=torch.float64, requires_grad=True) for x in theta]
+            flow = R.solve_stereo(r0, ow, masks, trs, sizes, *th, n_warp, n_iter, f_scale=spec.get("fScale", 2.0))
+            R.depth_loss(flow, pair["cam"], t, masks[0], z_gt).backward()
+            return time.perf_counter() - t0
+
+        run(1, 1)
+        pts = [(wn, n, run(wn, n)) for wn, n in ((1, 2), (1, 6), (2, 2), (2, 6))]
+    finally:
+        torch.set_num_threads(prev)
+    A_ = np.array([[1.0, wn, wn * n] for wn, n, _ in pts])
+    a_, b, d = np.linalg.lstsq(A_, np.array([t_ for _, _, t_ in pts]), rcond=None)[0]
+    full = float(a_ + spec["nWarpIters"] * (b + spec["nSolverIters"] * d))
+    runs = ", ".join(f"{wn} x {n}: {t_:.1f} s" for wn, n, t_ in pts)
+    return dict(value=1.0 / full, unit="pair gradients/s", cores=nt, kind="port",
+                sample=(f"float64 restatement of the solve (oracle/vfs_torch_ref.py) with torch autograd on {nt} "
+                        f"threads, one {hh}x{ww} pair, {spec['nLevel']} levels, warps x iterations per level "
+                        f"({runs}), fitted to A + w (B + n D) (A {a_:.2f} s, B {b:.2f} s, D {d:.3f} s) and "
+                        f"extrapolated to {spec['nWarpIters']} warps x {spec['nSolverIters']} iterations: {full:.0f} s"),
+                host=host)
+
+
+def adjoint_leg(n_pairs, steps, warmup, seed=0, compare=True, roofline=True, cpu_baseline=True):
     """Gradient of the depth loss (lambda, alpha0, alpha1, beta, gamma, reference image) for
-    `pairs` seeded 800x800 pairs, 3 levels x 10 warps x 50 iterations, checkpoint every 25.
+    `n_pairs` seeded 800x800 pairs, 3 levels x 10 warps x 50 iterations, checkpoint every 25.
     A step is forward + backward of every pair.  The plain (per-kernel, graph) and blocked
     forward solves of the same schedule are timed beside it for the cost ratio."""
     import torch
@@ -419,11 +499,11 @@ def run_adjoint(a):
 
     name = "adjoint_800"
     w = S.WORKLOADS[name]
-    n_pairs = min(a.pairs, w["pairs"]) if a.pairs else w["pairs"]
-    log(f"[bench] workload {name}: {n_pairs} pairs")
+    n_pairs = min(n_pairs, w["pairs"]) if n_pairs else w["pairs"]
+    log(f"[bench] adjoint: {n_pairs} pairs")
     ctxs, adjs = [], []
     for i in range(n_pairs):
-        ctx = setup_solver(name, a.seed + i, "graph", device="cuda")
+        ctx = setup_solver(name, seed + i, "graph", device="cuda")
         pr = ctx["pair"]
         adjs.append(A.VFSDepthAdjoint(ctx["prm"], pr["mask"], ctx["tvf"], ctx["cvf"], pr["cam"], pr["t"],
                                       pr["z_gt"], checkpoint=w["checkpoint"]))
@@ -441,20 +521,21 @@ def run_adjoint(a):
     v = ctxs[0]["vfs"]
     res = {}
     t_fwd = t_bwd = None
-    if not a.no_compare:  # --no-compare: only the timed steps (e.g. under a counter run)
+    if compare:
         for solver in ("graph", "blocked"):
             v.solver = solver
-            for _ in range(a.warmup):
+            for _ in range(warmup):
                 v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"])
-            res[solver] = clock(lambda: v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"]), max(a.steps, 3))
-            log(f"[bench] forward solve ({solver}): {1e3 * res[solver]:.2f} ms")
+            res[solver] = clock(lambda: v.solve_stereo(ctxs[0]["img_ref"], ctxs[0]["img_other"]), max(steps, 3))
+            log(f"[bench] adjoint: forward solve ({solver}): {1e3 * res[solver]:.2f} ms")
+        v.release_plan()
         adj = adjs[0]
-        for _ in range(a.warmup):
+        for _ in range(warmup):
             adj.forward(refs0[0], ctxs[0]["img_other"])
             adj.backward()
-        t_fwd = clock(lambda: adj.forward(refs0[0], ctxs[0]["img_other"]), max(a.steps, 3))
-        t_bwd = clock(adj.backward, max(a.steps, 3))
-        log(f"[bench] adjoint forward (with saves) {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms")
+        t_fwd = clock(lambda: adj.forward(refs0[0], ctxs[0]["img_other"]), max(steps, 3))
+        t_bwd = clock(adj.backward, max(steps, 3))
+        log(f"[bench] adjoint: forward (with saves) {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms")
 
     losses, grads = [], []
 
@@ -466,38 +547,52 @@ def run_adjoint(a):
             g = adjs[i].backward()
             grads.append([g[k] for k in A.PARAMS])
 
-    log(f"[bench] timing {a.steps} steps of {n_pairs} pair gradients")
-    for _ in range(a.warmup):
+    log(f"[bench] adjoint: timing {steps} steps of {n_pairs} pair gradients")
+    for _ in range(warmup):
         step()
-    dt = clock(step, a.steps)
+    dt = clock(step, steps)
     gmean = np.mean(np.asarray(grads), axis=0)
-    out = {
-        "metric": METRIC,
-        "value": round(n_pairs / dt, 3),
-        "unit": "pair gradients/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(1e3 * dt, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": name, "size": [w["H"], w["W"]], "levels": w["nLevel"],
-                   "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
-                   "checkpoint": w["checkpoint"], "pairs": n_pairs},
-        "forward_solve_ms": {k: round(1e3 * t, 3) for k, t in res.items()},
-        "adjoint_forward_ms": None if t_fwd is None else round(1e3 * t_fwd, 3),
-        "adjoint_backward_ms": None if t_bwd is None else round(1e3 * t_bwd, 3),
-        "backward_vs_plain_forward": None if t_bwd is None else round(t_bwd / res["graph"], 3),
-        "gradient_vs_plain_forward": None if t_bwd is None else round((t_fwd + t_bwd) / res["graph"], 3),
-        "gradient_vs_blocked_forward": None if t_bwd is None else round((t_fwd + t_bwd) / res["blocked"], 3),
-        "mean_loss": round(float(np.mean(losses)), 6),
-        "mean_gradient": {k: float(x) for k, x in zip(A.PARAMS, gmean)},
-        "roofline": None,
-        "cpu_baseline": None,
-    }
+    roof = None
+    if roofline:
+        log("[bench] adjoint: reverse-sweep kernel roofline pass (HIP events)")
+        r = adjoint_roofline(adjs[0], refs0[0], ctxs[0]["img_other"], 4096)
+        roof = dict(bound="hbm", kernel="k_adj_block", achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
+                    launches=r["launches"], tile=r["tile"], size=r["size"], units_per_launch=r["units"])
+    cpu = None
+    if cpu_baseline:
+        log("[bench] adjoint: CPU baseline (float64 restatement with autograd)")
+        pr = ctxs[0]["pair"]
+        cpu = cpu_adjoint_baseline(pr, ctxs[0]["tvf"], np.asarray(ctxs[0]["cvf"], np.float32), w)
+    for c in ctxs:
+        c["vfs"].release_plan()
+    return dict(value=round(n_pairs / dt, 3), unit="pair gradients/s", pairs=n_pairs, steps=steps,
+                ms_per_step=round(1e3 * dt, 3),
+                config={"workload": name, "size": [w["H"], w["W"]], "levels": w["nLevel"],
+                        "warps_per_level": w["nWarpIters"], "pd_iters_per_warp": w["nSolverIters"],
+                        "checkpoint": w["checkpoint"], "pairs": n_pairs},
+                forward_solve_ms={k: round(1e3 * t, 3) for k, t in res.items()},
+                adjoint_forward_ms=None if t_fwd is None else round(1e3 * t_fwd, 3),
+                adjoint_backward_ms=None if t_bwd is None else round(1e3 * t_bwd, 3),
+                backward_vs_plain_forward=None if t_bwd is None else round(t_bwd / res["graph"], 3),
+                gradient_vs_plain_forward=None if t_bwd is None else round((t_fwd + t_bwd) / res["graph"], 3),
+                gradient_vs_blocked_forward=None if t_bwd is None else round((t_fwd + t_bwd) / res["blocked"], 3),
+                mean_loss=round(float(np.mean(losses)), 6),
+                mean_gradient={k: float(x) for k, x in zip(A.PARAMS, gmean)},
+                roofline=roof, cpu_baseline=cpu)
+
+
+def run_adjoint(a):
+    """The gradient workload alone (--workload adjoint_800)."""
+    r = adjoint_leg(a.pairs, a.steps, a.warmup, seed=a.seed, compare=not a.no_compare,
+                    roofline=not a.no_roofline, cpu_baseline=not a.no_cpu_baseline)
+    out = {"metric": METRIC, "value": r.pop("value"), "unit": r.pop("unit"), "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": r.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": r.pop("config")}
+    r.pop("pairs")
+    r.pop("steps")
+    out.update(r)
     print(json.dumps(out), flush=True)
 
 
@@ -619,13 +714,15 @@ def main():
     ap.add_argument("--workload", default="plain_full_800",
                     choices=["plain_full_800", "plain_small_256", "default_800", "batched_64", "adjoint_800",
                              "fit_512"])
-    ap.add_argument("--pairs", type=int, default=64, help="batched_64: number of pairs (first n of the 64)")
+    ap.add_argument("--pairs", type=int, default=None,
+                    help="batched_64 / adjoint_800: number of pairs (the first n of the workload's 64 / 8)")
     ap.add_argument("--solver", default="blocked", choices=["graph", "eager", "blocked", "blocked_eager"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain per-kernel path beside it")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-pair batched leg of the default run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient leg of the default run")
     ap.add_argument("--save-flow", default=None, help="write the last step's flow (H, W, 2) to this .npy file")
     ap.add_argument("--probe", action="store_true",
                     help="only the launches of the hardware-counter runs (copy kernel + blocked kernel per level)")
@@ -721,12 +818,18 @@ def main():
         log("[bench] CPU oracle baseline")
         cpu = cpu_oracle_baseline(ctx, a.workload)
 
-    # the batched workload beside the headline: one timed batch of the 64 pairs
-    batched = None
+    # the batched workload beside the headline: one timed batch of the 64 pairs; and the
+    # gradient workload on 2 of its 8 pairs
+    batched = adjoint = None
     if a.workload == "plain_full_800" and not a.no_batch:
         v.release_plan()
         batched = batch_leg(64, steps=1, warmup=1, compare=not a.no_compare, roofline=not a.no_roofline,
                             cpu_baseline=not a.no_cpu_baseline)
+    if a.workload == "plain_full_800" and not a.no_adjoint:
+        v.release_plan()
+        torch.cuda.empty_cache()
+        adjoint = adjoint_leg(2, steps=2, warmup=1, seed=a.seed, compare=not a.no_compare,
+                              roofline=not a.no_roofline, cpu_baseline=not a.no_cpu_baseline)
 
     out = {
         "metric": METRIC,
@@ -754,6 +857,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_64": batched,
+        "adjoint_800": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -445,12 +445,22 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
  *     The blocked modes keep the fused kernel for the first and last unit of a warp;
  *   reverse_warps: 0 = the exact reverse of every warp; r > 0 = only the last r warps
  *     of the level are reversed and their start state is held fixed (the adjoints of
- *     the level's initial u, u_bar, flow are returned as zero): a truncated gradient. */
+ *     the level's initial u, u_bar, flow are returned as zero): a truncated gradient;
+ *   probe_events, probe_units, probe_count, probe_max: an optional timing probe of the
+ *     temporally blocked reverse launches (NULL / 0 = none).  For the j-th such launch
+ *     of the call, j < probe_max, probe_events[2 j] and probe_events[2 j + 1] (hipEvent_t,
+ *     created by the caller with timing enabled) are recorded on the stream just before
+ *     and after it and probe_units[j] (host memory) receives its unit count; on return
+ *     *probe_count (host) holds the number of launches probed. */
 typedef struct vfs_adj_opts {
     int32_t fwd_tile_x, fwd_tile_y, fwd_k, fwd_threads;
     int32_t rec_tile_x, rec_tile_y, rec_k, rec_threads;
     int32_t sweep;
     int32_t reverse_warps;
+    void* const* probe_events;
+    int32_t* probe_units;
+    int32_t* probe_count;
+    int32_t probe_max;
 } vfs_adj_opts;
 
 /* Same results as vfs_solve_level_plain (the per-kernel warp loop of

@@ -315,6 +315,34 @@ def test_gradients_are_bitwise_reproducible(T, shape, levels, ratio):
     _assert_same_bits(g0, g1)
 
 
+def test_reverse_probe_times_the_blocked_launches_and_leaves_gradients_unchanged(T):
+    """The timing probe of backward(): events around each temporally blocked reverse
+    launch of the probed level, one unit count per launch (n_iter - 1 units per warp:
+    the warp's first and last units run in the fused kernel), the same gradient bits
+    as without the probe, and a cap on the number of launches probed."""
+    pr, tvf, cvf = _small_case()
+    n_warp, n_iter = 2, 9
+    adj = _adjoint(pr, tvf, cvf, n_warp, n_iter, ckpt=4)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    adj.forward(ref0, pr["img_other"])
+    g0 = adj.backward()
+    adj.forward(ref0, pr["img_other"])
+    probe = dict(level=0, launches=1000)
+    g1 = adj.backward(probe=probe)
+    _assert_same_bits(g0, g1)
+    tx, ty, k, nt = probe["tiles"]
+    print("probe", probe["tiles"], probe["units"], [round(x, 2) for x in probe["us"]])
+    assert len(probe["us"]) == len(probe["units"]) > 0
+    assert all(x > 0 for x in probe["us"])
+    assert all(1 <= u <= k for u in probe["units"])
+    assert sum(probe["units"]) == n_warp * (n_iter - 1)
+    adj.forward(ref0, pr["img_other"])
+    capped = dict(level=0, launches=2)
+    g2 = adj.backward(probe=capped)
+    _assert_same_bits(g0, g2)
+    assert capped["units"] == probe["units"][:2]
+
+
 @pytest.mark.parametrize("shape,ckpt,rows", [((54, 70), 4, None), ((48, 48), 9, None), ((54, 70), 4, 1),
                                              ((54, 70), 5, 2), ((54, 70), 3, 4), ((100, 68), 4, 4), ((17, 23), 4, 1)])
 def test_fused_sweep_matches_one_kernel_per_step(T, shape, ckpt, rows):

@@ -1251,6 +1251,12 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     const int rw = tiles ? tiles->reverse_warps : 0;
     if (rw < 0) return VFS_EINVAL;
     const int w_end = (rw > 0 && rw < n_warp) ? n_warp - rw : 0;
+    // optional timing probe of the blocked reverse launches
+    const int probe_max = (tiles && tiles->probe_events) ? tiles->probe_max : 0;
+    if (probe_max < 0 || (probe_max > 0 && (tiles->probe_units == nullptr || tiles->probe_count == nullptr)))
+        return VFS_EINVAL;
+    int probed = 0;
+    if (probe_max > 0) *tiles->probe_count = 0;
     for (int w = n_warp - 1; w >= w_end; --w) {
         const WarpSave ws = warp_save_at(sv, n, n_iter, ckpt, w);
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, ws.flow, H, W, lv->tvf, H, W));
@@ -1434,7 +1440,19 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                     ba.eta_p = prm->eta_p;
                     ba.eta_q = prm->eta_q;
                     ba.lambda = prm->lambda;
+                    if (probed < probe_max) {
+                        hipEvent_t e0 = static_cast<hipEvent_t>(tiles->probe_events[2 * probed]);
+                        const hipError_t er = hipEventRecord(e0, s);
+                        if (er != hipSuccess) return static_cast<int>(er);
+                    }
                     VFS_TRY(launch_adj_block(*rv, s, rtiles, ba));
+                    if (probed < probe_max) {
+                        hipEvent_t e1 = static_cast<hipEvent_t>(tiles->probe_events[2 * probed + 1]);
+                        const hipError_t er = hipEventRecord(e1, s);
+                        if (er != hipSuccess) return static_cast<int>(er);
+                        tiles->probe_units[probed] = kc;
+                        *tiles->probe_count = ++probed;
+                    }
                     cur = 1 - cur;
                     scur = 1 - scur;
                     i -= kc;

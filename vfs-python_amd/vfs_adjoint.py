@@ -34,6 +34,12 @@ import vfs_hip as H
 
 PARAMS = ("Lambda", "alpha0", "alpha1", "beta", "gamma")
 
+# temporally blocked reverse sweeps: mode -> (tile_x, tile_y, units per launch, threads)
+# (kRevVariants of csrc/vfs_adjoint_solver.hip; 41-43 are the lean forms of 31-33)
+_REV_TILES = {21: (30, 30, 1, 1024), 22: (28, 28, 2, 1024), 23: (26, 26, 3, 1024), 24: (24, 24, 4, 1024),
+              25: (22, 22, 5, 1024), 26: (20, 20, 6, 1024), 31: (30, 14, 1, 512), 32: (28, 12, 2, 512),
+              33: (26, 10, 3, 512), 41: (30, 14, 1, 512), 42: (28, 12, 2, 512), 43: (26, 10, 3, 512)}
+
 
 def _z(shape, ch=None, dtype=torch.float32):
     s = tuple(shape) if ch is None else (*shape, ch)
@@ -260,8 +266,13 @@ class VFSDepthAdjoint:
         return self.loss
 
     # -- reverse --------------------------------------------------------------------------------
-    def backward(self):
-        """Returns dict(ref0=(H, W) tensor, Lambda=..., alpha0=..., alpha1=..., beta=..., gamma=...)."""
+    def backward(self, probe=None):
+        """Returns dict(ref0=(H, W) tensor, Lambda=..., alpha0=..., alpha1=..., beta=..., gamma=...).
+
+        probe: optional dict(level=k, launches=n), a timing hook: the first n temporally
+        blocked reverse launches of level k are bracketed by HIP events on the stream; on
+        return probe["us"] holds their durations (microseconds), probe["units"] their unit
+        counts and probe["tiles"] the launch variant (tile_x, tile_y, units, threads)."""
         if not self._done_forward:
             raise RuntimeError("backward() needs a forward() with the current parameters first")
         s = self.s
@@ -279,12 +290,16 @@ class VFSDepthAdjoint:
         # truncated: the finest level's start state is held fixed, so nothing reaches the
         # coarser levels (their reverse would run on zero adjoints)
         n_rev = 1 if self.reverse_warps > 0 else nL
+        held = None
         for k in range(n_rev):
             hh, ww = sizes[k]
             lv, st = b["level"][k]
+            opts = self._tiles(k)
+            if probe is not None and int(probe.get("level", 0)) == k:
+                held = self._probe_attach(opts, int(probe["launches"]))
             H.call("vfs_adj_reverse_level", s, ctypes.byref(st), ctypes.byref(prm), tp, sp, mp, n_iter, n_warp,
                    self.checkpoint, H.ptr(b["save"][k]), H.ptr(self._ws()), H.ptr(b["gu"][k]), H.ptr(b["gub"][k]),
-                   H.ptr(b["gf"][k]), H.ptr(b["gref"][k]), H.ptr(b["gpar"][k]), ctypes.byref(self._tiles(k)))
+                   H.ptr(b["gf"][k]), H.ptr(b["gref"][k]), H.ptr(b["gpar"][k]), ctypes.byref(opts))
             # level start: u = u_bar = fScale up(u_coarse), flow = fScale up(flow_coarse); the
             # coarse level's final u_bar is discarded, so its adjoint stays zero
             if k < n_rev - 1:
@@ -300,7 +315,31 @@ class VFSDepthAdjoint:
         tot = sum(self._reduce(g, 5) for g in b["gpar"]).cpu().numpy()
         out = {name: float(tot[i]) for i, name in enumerate(PARAMS)}
         out["ref0"] = b["gref"][0].clone()
+        if held is not None:
+            torch.cuda.synchronize()
+            evs, units, count, (opts, _) = held
+            n = int(count.value)
+            probe["us"] = [1e3 * evs[2 * j].elapsed_time(evs[2 * j + 1]) for j in range(n)]
+            probe["units"] = [int(units[j]) for j in range(n)]
+            probe["tiles"] = _REV_TILES.get(int(opts.sweep))
         return out
+
+    @staticmethod
+    def _probe_attach(opts, n):
+        """Fills the timing-probe fields of opts with 2 n timing events (recorded once on
+        the current stream so that their HIP handles exist) and host arrays; returns what
+        must stay alive until the events are read."""
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n)]
+        for e in evs:
+            e.record()
+        handles = (ctypes.c_void_p * (2 * n))(*[ctypes.c_void_p(e.cuda_event) for e in evs])
+        units = (ctypes.c_int32 * n)()
+        count = ctypes.c_int32(0)
+        opts.probe_events = ctypes.cast(handles, ctypes.POINTER(ctypes.c_void_p))
+        opts.probe_units = ctypes.cast(units, ctypes.POINTER(ctypes.c_int32))
+        opts.probe_count = ctypes.pointer(count)
+        opts.probe_max = n
+        return evs, units, count, (opts, handles)
 
 
 class DepthLoss(torch.autograd.Function):

@@ -56,7 +56,9 @@ class VfsAdjOpts(ctypes.Structure):
     """struct vfs_adj_opts (include/vfs_hip.h)."""
 
     _fields_ = [(n, ctypes.c_int32) for n in ("fwd_tile_x", "fwd_tile_y", "fwd_k", "fwd_threads", "rec_tile_x",
-                                              "rec_tile_y", "rec_k", "rec_threads", "sweep", "reverse_warps")]
+                                              "rec_tile_y", "rec_k", "rec_threads", "sweep", "reverse_warps")] + [
+        ("probe_events", ctypes.POINTER(ctypes.c_void_p)), ("probe_units", ctypes.POINTER(ctypes.c_int32)),
+        ("probe_count", ctypes.POINTER(ctypes.c_int32)), ("probe_max", ctypes.c_int32)]
 
 
 class VfsAdjPd(ctypes.Structure):
