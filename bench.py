@@ -209,6 +209,29 @@ def measured_traffic(per_level):
     return round(tot_b / tot_n)
 
 
+def measured_adj_traffic(tile):
+    """HBM bytes per k_adj_block launch of the given variant at the 800x800 gradient
+    workload: FETCH_SIZE + WRITE_SIZE (each scaled by its reading for a plain copy of
+    known size) averaged over the variant's launches of one gradient, from the counter
+    table in profiles/kernels_adjoint_800_hbm_lds.md.  None if the table has no single
+    row for the variant."""
+    path = os.path.join(ROOT, "profiles", "kernels_adjoint_800_hbm_lds.md")
+    if not os.path.exists(path):
+        return None
+    head = "| k_adj_block<" + ", ".join(str(x) for x in tile)
+    rows, cols = [], None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cols is None and "HBM MB / call" in cells:
+                cols = cells
+            elif line.startswith(head):
+                rows.append(cells)
+    if cols is None or len(rows) != 1:
+        return None
+    return round(float(rows[0][cols.index("HBM MB / call")]) * 1e6)
+
+
 def batched_kernel_roofline(batch):
     """Average duration and algorithmic bandwidth of the batched blocked primal-dual kernel
     over one batch solve's launches.  Per level, one warp of the batch's blocked launches
@@ -557,7 +580,8 @@ def adjoint_leg(n_pairs, steps, warmup, seed=0, compare=True, roofline=True, cpu
         log("[bench] adjoint: reverse-sweep kernel roofline pass (HIP events)")
         r = adjoint_roofline(adjs[0], refs0[0], ctxs[0]["img_other"], 4096)
         roof = dict(bound="hbm", kernel="k_adj_block", achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                    traffic=measured_adj_traffic(r["tile"]) if r["size"] == [800, 800] else None,
                     avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
                     launches=r["launches"], tile=r["tile"], size=r["size"], units_per_launch=r["units"])
     cpu = None
