@@ -121,3 +121,45 @@ def test_batch_at_default_parameters_equals_single_solves(T):
         single = v.solve_stereo(c["pair"]["img_ref"], c["pair"]["img_other"]).cpu().numpy()
         np.testing.assert_array_equal(f, single)
         assert np.isfinite(f).all() and np.abs(f).max() > 0
+
+
+def test_batch_of_more_than_64_pairs_equals_single_solves(T):
+    """70 small pairs in one batch: the launches' lookup of the pair that owns a tile or
+    a pixel block scans the descriptors 64 at a time, so pairs 64-69 sit in the second
+    group; pairs on both sides of that boundary equal their single-pair solves bit for
+    bit."""
+    import copy
+    import synthetic as S
+    import vfs as V
+    import vfs_batch as B
+    shapes = [(20, 24), (24, 20), (17, 23), (22, 22), (19, 26)]
+    sizes = [shapes[i % len(shapes)] for i in range(70)]
+    pairs = []
+    for i, (h, w) in enumerate(sizes):
+        pr = S.make_pair(h, w, seed=300 + i, t=(0.2, 0.0, 0.02), rot_deg=1.5, rot_random=True, n_levels=2)
+        tvf, cvf = O.get_vector_fields(O.get_grid_zero((h, w)), pr["cam"], pr["cam"], pr["R"], pr["t"], 1)
+        pairs.append(dict(pair=pr, tvf=S.sanitize_field(tvf), cvf=cvf.astype(np.float32)))
+    prm = _params()
+    prm.nWarpIters, prm.nSolverIters = 2, 6
+    batch = B.VFSBatch(prm, sizes)
+    batch.pd_tiles = TILES
+    for i, c in enumerate(pairs):
+        batch.setup_pair(i, c["pair"]["mask"], c["tvf"], c["cvf"])
+    flows = [f.cpu().numpy() for f in batch.solve([c["pair"]["img_ref"] for c in pairs],
+                                                  [c["pair"]["img_other"] for c in pairs])]
+    assert all(np.isfinite(f).all() for f in flows)
+    for i in (0, 1, 62, 63, 64, 65, 68, 69):
+        (h, w), c = sizes[i], pairs[i]
+        p = copy.copy(prm)
+        p.height, p.width = h, w
+        v = V.VFS(p)
+        v.solver = "blocked"
+        v.pd_tiles = TILES
+        v.setup_module()
+        v.compute_pyramid_sizes()
+        v.create_mask_pyramid(c["pair"]["mask"])
+        v.create_vector_field_pyramid(c["tvf"], c["cvf"])
+        single = v.solve_stereo(c["pair"]["img_ref"], c["pair"]["img_other"]).cpu().numpy()
+        print(i, (h, w), "max |batch - single|", float(np.abs(flows[i] - single).max()))
+        np.testing.assert_array_equal(flows[i], single)
+        assert np.abs(flows[i]).max() > 0
