@@ -189,6 +189,33 @@ def run_fit(a):
     print(json.dumps(out), flush=True)
 
 
+_COPY_GBS = {}
+
+
+def stream_copy_gbs(n=1 << 26, reps=5):
+    """A measured stream-copy rate of the device: vfs_copy of n floats (read + write,
+    8 n bytes), best of `reps` launches timed with HIP events on the launch stream
+    (measured once per process)."""
+    if n in _COPY_GBS:
+        return _COPY_GBS[n]
+    import torch
+    import vfs_hip as H
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    st = torch.cuda.current_stream()
+    best = float("inf")
+    for _ in range(reps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        H.call("vfs_copy", H.stream(), H.ptr(dst), H.ptr(src), n)
+        e1.record(st)
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    _COPY_GBS[n] = 8.0 * n / (best * 1e-3) / 1e9
+    return _COPY_GBS[n]
+
+
 def measured_traffic(per_level):
     """HBM bytes per k_pd_block launch, averaged over the launches of one solve like
     `achieved`, from the FETCH_SIZE / WRITE_SIZE runs recorded in
@@ -382,9 +409,11 @@ def batch_leg(n_pairs=64, steps=1, warmup=1, compare=True, roofline=True, cpu_ba
     if roofline:
         log("[bench] batch: batched-kernel roofline pass (HIP events)")
         r = batched_kernel_roofline(batch)
+        copy_gbs = stream_copy_gbs()
         roof = dict(bound="hbm", kernel="k_pd_block_batched", achieved=round(r["achieved_gbs"], 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None,
                     avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
+                    stream_copy_gbs=round(copy_gbs, 1), frac_of_stream_copy=round(r["achieved_gbs"] / copy_gbs, 4),
                     per_level=r["per_level"])
     cpu = None
     if cpu_baseline:
@@ -583,7 +612,8 @@ def adjoint_leg(n_pairs, steps, warmup, seed=0, compare=True, roofline=True, cpu
                     unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
                     traffic=measured_adj_traffic(r["tile"]) if r["size"] == [800, 800] else None,
                     avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
-                    launches=r["launches"], tile=r["tile"], size=r["size"], units_per_launch=r["units"])
+                    launches=r["launches"], tile=r["tile"], size=r["size"], units_per_launch=r["units"],
+                    stream_copy_gbs=round(stream_copy_gbs(), 1))
     cpu = None
     if cpu_baseline:
         log("[bench] adjoint: CPU baseline (float64 restatement with autograd)")
@@ -832,9 +862,11 @@ def main():
             r = dual_kernel_roofline(sizes, w["nWarpIters"], w["nSolverIters"], v.param)
             kname = "k_update_dual"
         traffic = measured_traffic(r["per_level"]) if kname == "k_pd_block" else None
+        copy_gbs = stream_copy_gbs()
         roof = dict(bound="hbm", kernel=kname, achieved=round(r["achieved_gbs"], 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                     avg_launch_us=round(r["avg_us"], 3), bytes_per_launch=round(r["bytes_per_launch"]),
+                    stream_copy_gbs=round(copy_gbs, 1), frac_of_stream_copy=round(r["achieved_gbs"] / copy_gbs, 4),
                     per_level=r["per_level"])
 
     cpu = None

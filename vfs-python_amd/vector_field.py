@@ -48,11 +48,14 @@ def get_vector_fields(grid, camera_ref, camera_other, R, t, t_scale, normalizer)
 
 
 def get_grid_zero(sz):
-    """Python/vector_field.py:58-62: (H, W, 2) float32 pixel grid (x, y) on the GPU."""
-    uv = np.empty((*sz, 2), dtype=np.int32)
-    uv[:, :, 0] = np.arange(0, sz[1])[np.newaxis, :]
-    uv[:, :, 1] = np.arange(0, sz[0])[:, np.newaxis]
-    return torch.as_tensor(uv, dtype=torch.float32, device="cuda")
+    """Python/vector_field.py:58-62: (H, W, 2) float32 pixel grid (x, y) on the GPU.
+    Built on the device (integers below 2^24 are exact in float32), so no host array is
+    copied over."""
+    h, w = int(sz[0]), int(sz[1])
+    uv = torch.empty((h, w, 2), dtype=torch.float32, device="cuda")
+    uv[:, :, 0] = torch.arange(w, dtype=torch.float32, device="cuda")[None, :]
+    uv[:, :, 1] = torch.arange(h, dtype=torch.float32, device="cuda")[:, None]
+    return uv
 
 
 def get_grid_one(sz):
