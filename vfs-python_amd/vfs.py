@@ -84,6 +84,7 @@ class VFS:
         self._plan = None
         self._plan_key = None
         self._bufs = None
+        self._t_dev = None  # (translation values, their device copy) of the last triangulation
 
     # -- module setup (Python/vfs.py:38-74) ---------------------------------------------
     def compile_module(self):
@@ -552,7 +553,12 @@ class VFS:
         rays_other = camera.get_XYZs((uv + self.warpUV).contiguous())
         assert rays.is_contiguous() and rays_other.is_contiguous()
         assert rays.dtype == torch.float32 and rays_other.dtype == torch.float32
-        translation_ = _f32_device(np.asarray(translation, dtype=np.float32).reshape(-1))
+        # the translation's device copy is kept per value: a copy from host memory makes
+        # torch wait for the stream, which would stall the host behind the solve
+        tkey = tuple(float(x) for x in np.asarray(translation, dtype=np.float32).reshape(-1))
+        if self._t_dev is None or self._t_dev[0] != tkey:
+            self._t_dev = (tkey, _f32_device(np.asarray(tkey, dtype=np.float32)))
+        translation_ = self._t_dev[1]
         d = torch.empty_like(rays)
         m = self.masks_to[0].require(texture.cudaAddressModeMirror)
         H.call("vfs_triangulate_rays", H.stream(), H.ptr(d), H.ptr(rays), H.ptr(rays_other), H.ptr(translation_),
