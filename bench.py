@@ -336,6 +336,7 @@ def batch_leg(n_pairs=64, steps=1, warmup=1, compare=True, roofline=True, cpu_ba
     import vector_field as VF
     import equidistant_cam as E
     spec = S.BATCH_64
+    n_pairs = min(n_pairs, len(spec["sizes"])) if n_pairs else len(spec["sizes"])
     log(f"[bench] batch: generating {n_pairs} pairs on the GPU")
     pairs = S.batch_pairs(spec, device="cuda", n=n_pairs)
     sizes = [p["img_ref"].shape for p in pairs]
