@@ -354,6 +354,14 @@ __global__ void __launch_bounds__(256) k_copy(float* __restrict__ out, const flo
         out[i] = in[i];
 }
 
+// The same copy 16 bytes per lane (out and in 16-byte aligned); n4 = n / 4 vectors.
+__global__ void __launch_bounds__(256) k_copy4(float4* __restrict__ out, const float4* __restrict__ in, int64_t n4)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n4;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        out[i] = in[i];
+}
+
 __global__ void __launch_bounds__(256) k_zero_masked(float* __restrict__ flow, const uint8_t* __restrict__ mask,
                                                      int H, int W, int channels)
 {
@@ -620,7 +628,14 @@ int vfs_copy(void* stream, float* out, const float* in, int64_t n)
     if (n == 0) return 0;
     VFS_CHECK_PTR(out);
     VFS_CHECK_PTR(in);
-    hipLaunchKernelGGL(k_copy, dim3(blocks1d(n)), dim3(256), 0, S(stream), out, in, n);
+    const bool aligned = ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(in)) & 15u) == 0;
+    const int64_t n4 = aligned ? n / 4 : 0;
+    if (n4 > 0)
+        hipLaunchKernelGGL(k_copy4, dim3(blocks1d(n4)), dim3(256), 0, S(stream), reinterpret_cast<float4*>(out),
+                           reinterpret_cast<const float4*>(in), n4);
+    if (n - 4 * n4 > 0)
+        hipLaunchKernelGGL(k_copy, dim3(blocks1d(n - 4 * n4)), dim3(256), 0, S(stream), out + 4 * n4, in + 4 * n4,
+                           n - 4 * n4);
     VFS_LAUNCH_STATUS();
 }
 

@@ -173,7 +173,7 @@ def pmc(fetch_dir, write_dir, probe_log, n_iter):
             levels.append(((h, w), (tx, ty, k, nt)))
 
     def copies(p):
-        return sorted(v[2] for v in p if v[0] == "k_copy" and v[1] == COPY_FLOATS // 4)
+        return sorted(v[2] for v in p if v[0] in ("k_copy", "k_copy4") and v[1] == COPY_FLOATS // 4)
 
     cf, cw = copies(fe), copies(wr)
     if not cf or not cw:
