@@ -10,9 +10,10 @@ synchronisations.
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
 (measured with HIP events on the launch stream), and the float64 CPU oracle
-timed on the host beside it.  The default run adds two legs to the same line: the
+timed on the host beside it.  The default run adds three legs to the same line: the
 64-pair batched solve ("batched_64") and the depth-loss gradient on two 800x800
-pairs ("adjoint_800"), each with its own roofline and CPU baseline.
+pairs ("adjoint_800"), each with its own roofline and CPU baseline, and the scaled
+parameter fit ("fit_512").
 """
 
 import argparse
@@ -151,41 +152,39 @@ def blocked_kernel_roofline(sizes, tiles, n_warp, n_iter, levels=None):
                 achieved_gbs=bytes_per_launch / avg_s / 1e9, per_level=per_level)
 
 
-def run_fit(a):
+def fit_leg():
     """The scaled parameter fit (fit.FIT_DEMO): 16 pairs at 512x512, 12 Adam steps on the
     5 solver parameters from the reference's defaults through the adjoint (truncated
-    gradient); loss and median relative depth error of 8 held-out pairs before and after.  The timed region is the fit itself (--steps / --warmup do
-    not apply)."""
+    gradient); loss and median relative depth error of 8 held-out pairs before and after.
+    The timed region is the fit itself."""
     import fit
     r = fit.run_spec(dict(fit.FIT_DEMO), log=log)
-    out = {
-        "metric": METRIC,
-        "value": round(r["pair_gradients_per_s"], 3),
-        "unit": "pair gradients/s",
-        "n_gpus": 1,
-        "steps": r["spec"]["steps"],
-        "warmup": 0,
-        "ms_per_step": round(1e3 * r["fit_seconds"] / r["spec"]["steps"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "fit_512", "size": list(r["spec"]["size"]), "pairs": r["spec"]["pairs"],
-                   "pairs_per_step": r["spec"]["batch"], "adam_steps": r["spec"]["steps"],
-                   "held_out": r["spec"]["held_out"],
-                   "schedule": fit.SCHEDULE},
-        "fit_seconds": round(r["fit_seconds"], 3),
-        "start": r["start"],
-        "fitted": {k: round(v, 5) for k, v in r["fitted"].items()},
-        "gradient": "truncated (last warp of the finest level)" if r["spec"]["reverse_warps"] else "exact",
-        "held_out_loss": {"start": round(r["held_out_loss_start"], 5), "fitted": round(r["held_out_loss_fitted"], 5)},
-        "held_out_median_rel_depth_err": {"start": round(r["held_out_median_rel_depth_err_start"], 5),
-                                          "fitted": round(r["held_out_median_rel_depth_err_fitted"], 5)},
-        "train_loss": [round(x, 5) for x in r["train_loss"]],
-        "roofline": None,
-        "cpu_baseline": None,
-    }
+    return dict(
+        value=round(r["pair_gradients_per_s"], 3), unit="pair gradients/s", steps=r["spec"]["steps"],
+        ms_per_step=round(1e3 * r["fit_seconds"] / r["spec"]["steps"], 3),
+        config={"workload": "fit_512", "size": list(r["spec"]["size"]), "pairs": r["spec"]["pairs"],
+                "pairs_per_step": r["spec"]["batch"], "adam_steps": r["spec"]["steps"],
+                "held_out": r["spec"]["held_out"], "schedule": fit.SCHEDULE},
+        fit_seconds=round(r["fit_seconds"], 3),
+        start=r["start"],
+        fitted={k: round(v, 5) for k, v in r["fitted"].items()},
+        gradient="truncated (last warp of the finest level)" if r["spec"]["reverse_warps"] else "exact",
+        held_out_loss={"start": round(r["held_out_loss_start"], 5), "fitted": round(r["held_out_loss_fitted"], 5)},
+        held_out_median_rel_depth_err={"start": round(r["held_out_median_rel_depth_err_start"], 5),
+                                       "fitted": round(r["held_out_median_rel_depth_err_fitted"], 5)},
+        train_loss=[round(x, 5) for x in r["train_loss"]],
+        # the fit's kernels are the gradient leg's (forward blocked launches, reverse sweep);
+        # their roofline and CPU baseline are reported there
+        roofline=None, cpu_baseline=None)
+
+
+def run_fit(a):
+    """The fit alone (--workload fit_512; --steps / --warmup do not apply)."""
+    r = fit_leg()
+    out = {"metric": METRIC, "value": r.pop("value"), "unit": r.pop("unit"), "n_gpus": 1, "steps": r.pop("steps"),
+           "warmup": 0, "ms_per_step": r.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": r.pop("config")}
+    out.update(r)
     print(json.dumps(out), flush=True)
 
 
@@ -778,6 +777,7 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-pair batched leg of the default run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient leg of the default run")
+    ap.add_argument("--no-fit", action="store_true", help="skip the parameter-fit leg of the default run")
     ap.add_argument("--save-flow", default=None, help="write the last step's flow (H, W, 2) to this .npy file")
     ap.add_argument("--probe", action="store_true",
                     help="only the launches of the hardware-counter runs (copy kernel + blocked kernel per level)")
@@ -887,6 +887,10 @@ def main():
         torch.cuda.empty_cache()
         adjoint = adjoint_leg(2, steps=2, warmup=1, seed=a.seed, compare=not a.no_compare,
                               roofline=not a.no_roofline, cpu_baseline=not a.no_cpu_baseline)
+    fitted = None
+    if a.workload == "plain_full_800" and not a.no_fit:
+        torch.cuda.empty_cache()
+        fitted = fit_leg()
 
     out = {
         "metric": METRIC,
@@ -915,6 +919,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_64": batched,
         "adjoint_800": adjoint,
+        "fit_512": fitted,
     }
     print(json.dumps(out), flush=True)
 
