@@ -705,11 +705,13 @@ def _oracle_params(name_or_spec, hh, ww, n_warp=None):
     return prm
 
 
-def cpu_oracle_baseline(ctx, name):
+def cpu_oracle_baseline(ctx, name, gpu_flow=None, gpu_z=None):
     """The float64 C restatement of the oracle (oracle/vfs_oracle.c) on the host: one whole
     solve of the same pair and schedule plus triangulation, on every core of the process's
     affinity (OpenMP over image rows; the flow does not depend on the thread count), and a
-    bounded single-thread sample (1 warp per level, extrapolated)."""
+    bounded single-thread sample (1 warp per level, extrapolated).  Returns the baseline and,
+    given the timed GPU flow and depth, their deviation from the oracle's over the mask
+    (|du| in pixels; relative depth difference where both depths are finite and positive)."""
     from oracle import vfs_oracle as O
     from oracle import vfs_oracle_c as C
     import synthetic as S
@@ -721,19 +723,32 @@ def cpu_oracle_baseline(ctx, name):
     args = (pair["img_ref"], pair["img_other"], pair["mask"], ctx["tvf"], np.asarray(ctx["cvf"], np.float32))
     t0 = time.perf_counter()
     flow = C.solve_stereo(*args, _oracle_params(name, hh, ww), threads=nt)
-    O.triangulate_flow(flow, pair["cam"], pair["t"].reshape(3), pair["mask"])
+    xyz = O.triangulate_flow(flow, pair["cam"], pair["t"].reshape(3), pair["mask"])
     dt = time.perf_counter() - t0
     t1 = time.perf_counter()
     C.solve_stereo(*args, _oracle_params(name, hh, ww, n_warp=1), threads=1)
     dt1 = (time.perf_counter() - t1) * w["nWarpIters"]
-    return dict(value=1.0 / dt, unit="depth maps/s", cores=nt, kind="port",
-                sample=(f"float64 C restatement of the oracle (oracle/vfs_oracle.c), one whole {hh}x{ww} solve "
-                        f"({w['nLevel']} levels x {w['nWarpIters']} warps x {w['nSolverIters']} iterations) and "
-                        f"triangulation on {nt} threads: {dt:.1f} s"),
-                single_thread=dict(value=1.0 / dt1, unit="depth maps/s", cores=1,
-                                   sample=(f"1 thread, 1 of {w['nWarpIters']} warps per level: {dt1 / w['nWarpIters']:.1f} s, "
-                                           f"extrapolated x{w['nWarpIters']}")),
-                host=host)
+    vs = None
+    if gpu_flow is not None:
+        m = pair["mask"] > 0
+        d = np.linalg.norm(np.asarray(gpu_flow, np.float64) - flow, axis=-1)[m]
+        vs = dict(median_abs_du_px=float(np.median(d)), p99_abs_du_px=float(np.percentile(d, 99)),
+                  max_abs_du_px=float(d.max()))
+        if gpu_z is not None:
+            zo, zg = xyz[..., 2][m], np.asarray(gpu_z, np.float64)[m]
+            ok = np.isfinite(zo) & np.isfinite(zg) & (zo > 0) & (zg > 0)
+            vs["median_rel_depth_diff"] = float(np.median(np.abs(zg[ok] - zo[ok]) / zo[ok]))
+            vs["oracle_median_rel_depth_err_vs_gt"] = float(np.median(
+                np.abs(zo[ok] - pair["z_gt"][m][ok]) / pair["z_gt"][m][ok]))
+    cpu = dict(value=1.0 / dt, unit="depth maps/s", cores=nt, kind="port",
+               sample=(f"float64 C restatement of the oracle (oracle/vfs_oracle.c), one whole {hh}x{ww} solve "
+                       f"({w['nLevel']} levels x {w['nWarpIters']} warps x {w['nSolverIters']} iterations) and "
+                       f"triangulation on {nt} threads: {dt:.1f} s"),
+               single_thread=dict(value=1.0 / dt1, unit="depth maps/s", cores=1,
+                                  sample=(f"1 thread, 1 of {w['nWarpIters']} warps per level: {dt1 / w['nWarpIters']:.1f} s, "
+                                          f"extrapolated x{w['nWarpIters']}")),
+               host=host)
+    return cpu, vs
 
 
 def cpu_batch_baseline(pairs_by_size, spec):
@@ -870,10 +885,10 @@ def main():
                     stream_copy_gbs=round(copy_gbs, 1), frac_of_stream_copy=round(r["achieved_gbs"] / copy_gbs, 4),
                     per_level=r["per_level"])
 
-    cpu = None
+    cpu = vs_oracle = None
     if not a.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
-        cpu = cpu_oracle_baseline(ctx, a.workload)
+        cpu, vs_oracle = cpu_oracle_baseline(ctx, a.workload, flow_main, xyz[..., 2].cpu().numpy())
 
     # the batched workload beside the headline: one timed batch of the 64 pairs; and the
     # gradient workload on 2 of its 8 pairs
@@ -915,6 +930,7 @@ def main():
         "mpx_iter_per_s": round(px_iter * maps_s / 1e6, 1),
         "solver_effective_gbs": round(bytes_map * maps_s / 1e9, 1),
         "median_rel_depth_err_vs_gt": round(depth_err, 4),
+        "vs_oracle": vs_oracle,
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_64": batched,
