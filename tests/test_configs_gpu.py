@@ -6,7 +6,14 @@
   equals its single-pair blocked solve with the batch's tiles bit for bit; every flow
   is finite and exactly zero outside the mask; the median relative depth error
   against the ground truth is <= 0.05 in each size class and <= 0.1 for every pair
-  (measured 0.025-0.030 per class, at most 0.059).
+  (measured 0.025-0.030 per class, at most 0.059).  Against the whole float64 solve of
+  the oracle's C restatement (the first pair of each size, same inputs and vector
+  fields), over the mask: median |d flow| <= 1.5x and 99th percentile <= 3x those of
+  the float32 restatement of the same formulas (F32 below: the spread single-precision
+  rounding alone causes on that pair, measured once with oracle.vfs_oracle in float32
+  against the same float64 solve; these poses are less well conditioned than the
+  headline pair's, so the spread exceeds its 1.8e-4 px median), and median relative
+  depth difference <= 1e-3.  Measured GPU medians: 3.4e-3, 5.2e-4, 2.7e-4 and 1.4e-4 px.
 * Differentiation (800 x 800, 3 levels x 10 warps x 50 iterations, checkpoint every
   25 iterations): the loss and all gradients are finite; a checkpoint every 10
   iterations gives the same bits; a second forward and backward gives the same bits.
@@ -33,6 +40,35 @@ def T():
 def _err_vs_gt(z, pr):
     m = (pr["mask"] > 0) & np.isfinite(z) & (pr["z_gt"] > 1e-3)
     return float(np.median(np.abs(z[m] - pr["z_gt"][m]) / pr["z_gt"][m]))
+
+
+# float32 restatement vs float64 oracle on the first pair of each size class: (median, p99) of |d flow|, px
+F32 = {(512, 512): (3.16e-3, 5.83e-2), (800, 800): (5.44e-4, 6.36e-3), (1024, 1024): (2.56e-4, 3.47e-3),
+       (960, 1280): (1.60e-4, 3.77e-3)}
+
+
+def _check_against_oracle(p, field, flow, z, spec):
+    import os
+    import synthetic as S
+    from oracle import vfs_oracle as O
+    from oracle import vfs_oracle_c as C
+    hh, ww = p["img_ref"].shape
+    prm = O.OracleParams(hh, ww)
+    S.apply_workload_params(prm, spec)
+    tvf, cvf = (np.asarray(a.cpu().numpy() if hasattr(a, "cpu") else a, np.float32) for a in field)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    fo = C.solve_stereo(p["img_ref"], p["img_other"], p["mask"], tvf, cvf, prm, threads=threads)
+    zo = O.triangulate_flow(fo, p["cam"], np.asarray(p["t"]).reshape(3), p["mask"])[..., 2]
+    m = p["mask"] > 0
+    d = np.linalg.norm(flow.astype(np.float64) - fo, axis=-1)[m]
+    ok = m & np.isfinite(z) & np.isfinite(zo) & (z > 0) & (zo > 0)
+    rel = float(np.median(np.abs(z[ok] - zo[ok]) / zo[ok]))
+    print((hh, ww), "vs oracle: median |du|", float(np.median(d)), "p99", float(np.percentile(d, 99)),
+          "max", float(d.max()), "median rel depth diff", rel)
+    f32_median, f32_p99 = F32[(hh, ww)]
+    assert np.median(d) <= 1.5 * f32_median
+    assert np.percentile(d, 99) <= 3 * f32_p99
+    assert rel <= 1e-3
 
 
 def test_batched_64_pairs(T):
@@ -81,6 +117,7 @@ def test_batched_64_pairs(T):
             single = v.solve_stereo(p["img_ref"], p["img_other"])
             assert T.equal(single, f), i
             v.release_plan()
+            _check_against_oracle(p, fields[i], fn, x[..., 2].cpu().numpy(), spec)
     for k in range(4):
         e = errs[16 * k:16 * k + 16]
         print(sizes[16 * k], "median relative depth error", np.median(e), "max", np.max(e))
