@@ -306,3 +306,61 @@ def test_triangulation_and_distance(T):
     c.set_mask(mask)
     xd = _host(c.get_xyz_from_distance(_dev(T, D.astype(np.float32))))
     np.testing.assert_allclose(xd[m], P[m], rtol=2e-6, atol=2e-6)
+
+
+def test_derivatives_with_zero_length_translation_vectors(T):
+    """A zero translation vector has no direction (hypot 0, Python/vfs.cu:127-129): Iu is
+    non-finite there on both sides, Iz (a plain difference) stays finite, every other
+    pixel matches, and no sample index leaves the image."""
+    import texture
+    rng = np.random.default_rng(21)
+    Hh, Ww = 33, 41
+    v, _ = _vfs(Hh, Ww)
+    ref = rng.random((Hh, Ww)).astype(np.float32)
+    ow = rng.random((Hh, Ww)).astype(np.float32)
+    ang = rng.uniform(0, 2 * np.pi, (Hh, Ww))
+    tvf = np.stack([np.cos(ang), np.sin(ang)], -1).astype(np.float32)
+    zero = rng.random((Hh, Ww)) < 0.05
+    zero[0, 0] = zero[Hh - 1, Ww - 1] = zero[Hh // 2, Ww // 2] = True
+    tvf[zero] = 0.0
+    d = _host(v.compute_derivatives(_dev(T, ow), texture.wrap(_dev(T, ref)), texture.wrap(_dev(T, tvf))))
+    d_o = O.compute_derivatives(ow, ref, tvf)
+    assert np.array_equal(np.isfinite(d), np.isfinite(d_o))
+    assert not np.isfinite(d[..., 0][zero]).any() and np.isfinite(d[..., 1]).all()
+    ok = np.isfinite(d_o)
+    np.testing.assert_allclose(d[ok], d_o[ok], rtol=0, atol=2e-5)
+
+
+def test_triangulation_with_zero_disparity(T):
+    """Where the other view's ray equals the reference ray (zero disparity, Python/vfs.cu:733)
+    the depth has a zero denominator: the GPU gives the same non-finite values (sign
+    included) as the oracle there, and matches it elsewhere."""
+    import equidistant_cam as E
+    import texture
+    Hh, Ww = 48, 48
+    cam = (Ww / np.pi, Ww / 2.0, Hh / 2.0)
+    rng = np.random.default_rng(22)
+    g = O.get_grid_zero((Hh, Ww))
+    D = 2 + 3 * rng.random((Hh, Ww))
+    t = np.array([0.25, 0.0, 0.03])
+    P = D[..., None] * O.unproject(g, *cam)
+    flow = (O.project(P + t, *cam) - g).astype(np.float32)
+    still = rng.random((Hh, Ww)) < 0.1
+    flow[still] = 0.0
+    mask = ((np.hypot(g[..., 0] - Ww / 2, g[..., 1] - Hh / 2) < 20).astype(np.uint8) * 255)
+    v, _ = _vfs(Hh, Ww)
+    v.masks = [_dev(T, mask)]
+    v.masks_to = [texture.wrap(v.masks[0], texture.cudaFilterModePoint)]
+    v.warpUV = _dev(T, flow)
+    c = E.EquidistantCam(cam)
+    c.setup_module()
+    xyz = _host(v.triangulate_flow(c, t.reshape(3, 1))).astype(np.float64)
+    ref = O.triangulate_flow(flow.astype(np.float64), cam, t, mask)
+    m = mask > 0
+    assert (still & m).sum() > 10
+    z, zo = xyz[..., 2][m], ref[..., 2][m]
+    assert np.array_equal(np.isnan(z), np.isnan(zo))
+    assert np.array_equal(np.isposinf(z), np.isposinf(zo)) and np.array_equal(np.isneginf(z), np.isneginf(zo))
+    assert not np.isfinite(z[still[m]]).all()
+    ok = np.isfinite(zo)
+    np.testing.assert_allclose(z[ok], zo[ok], rtol=2e-4, atol=1e-4)
