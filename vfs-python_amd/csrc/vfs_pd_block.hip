@@ -582,6 +582,10 @@ const Variant kVariants[] = {
     VFS_PD_VARIANT(28, 12, 2, 512),  VFS_PD_VARIANT(26, 10, 3, 512),  VFS_PD_VARIANT(24, 8, 4, 512),
     VFS_PD_VARIANT(14, 14, 1, 256),  VFS_PD_VARIANT(12, 12, 2, 256),  VFS_PD_VARIANT(10, 10, 3, 256),
     VFS_PD_VARIANT(8, 8, 4, 256),
+    // coarse levels (one workgroup per CU there): a 27 x 27 region on 768 lanes puts three
+    // wavefronts on each SIMD instead of four, and 7 x 7 tiles still cover 100 x 100 with
+    // 225 workgroups
+    VFS_PD_VARIANT(7, 7, 10, 768),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
