@@ -362,8 +362,9 @@ int vfs_adj_clamp(void* stream, float* gu, float* guf, int* pos, float* gtv, con
                   float limit, int ksize);
 /* Reverse of medianFilter: gu_th (H,W) := adjoint of the thresholded u. */
 int vfs_adj_median(void* stream, float* gu_th, const float* guf, const int* pos, int H, int W, int ksize);
-/* Reverse of computeDerivatives (Python/vfs.cu:120-142): gref, gow (H,W) += (gather: each
- * texel sums its senders' tap shares in a fixed order, so the result is deterministic). */
+/* Reverse of computeDerivatives (Python/vfs.cu:120-142): gref, gow (H,W) += the texels'
+ * tap shares (each a float product, summed exactly in int64 fixed point per block and
+ * rounded once, so the result does not depend on the order of the additions). */
 int vfs_adj_derivatives(void* stream, float* gref, float* gow, const float* gderiv, const float* tvf,
                         int H, int W);
 /* Reverse of warpingImage / findWarpingVector at the warp-start flow:

@@ -302,22 +302,21 @@ __device__ __forceinline__ float tap_share(int r, int H, int W, float x, float y
 // the reference level image and the warped image by giu * (1, -8, 8, -1), giu =
 // g(Iu) / 24, and its centre by -g(Iz) (reference) and +g(Iz) (warped).  |k e| <= 2,
 // so a tap's corners lie within [s - 2, s + 3] per axis and, after the mirror,
-// every sender of texel r lies within r +- 3.  Gather: gref += T - g(Iz),
-// gow += T + g(Iz), with T the taps' shares at r.
+// every sender of texel r lies within r +- 3.  gref += T - g(Iz), gow += T + g(Iz),
+// with T the taps' shares at r.
 //
-// A block stages (giu, e) of its receivers' senders (the block extended by 3 on
-// every side) in LDS.  Where e varies little over a receiver's 7 x 7 senders
-// (every component within +-0.2 of the window's centre value) and the receiver is
-// at least 3 pixels from the image border (so no mirrored corner can fall on it),
-// only the senders whose tap k can reach it are visited: a corner is r when the tap
-// lies in [r - 1, r + 1), so s lies in [r - 1 - k e(s), r + 1 - k e(s)), at most
-// three integers per axis for |k| <= 2 and |e(s) - centre| <= 0.2 (checked exactly
-// per candidate).  Elsewhere (border pixels, directions that turn fast) every
-// sender of the window is visited.  Either way the shares are summed in a fixed
-// order, so the result is deterministic.
+// A block owns a 64 x 16 tile of texels and visits the senders of the tile extended by
+// 3 on every side.  Each sender forms its taps' corner shares in float as tap_share
+// does (v (1 - ax) (1 - ay), ...) and adds those that fall on the tile into int64
+// fixed-point sums in LDS: share x 2^S, rounded to the nearest integer, with S chosen
+// per block from the largest |giu| M of its senders so that one share is below 2^54
+// and a texel's at most 196 shares sum below 2^62.  Integer sums do not depend on the
+// order of the additions, so the result is deterministic; T is the sum rounded once
+// to float (shares below 2^-S-1, under 2^-57 M, round to zero).  A non-finite share
+// makes its texel's T NaN; taps at non-finite coordinates add nothing.
 namespace {
-constexpr int DR = 3, DTX = VFS_BLOCK_X, DTY = VFS_BLOCK_Y, DLX = DTX + 2 * DR, DLY = DTY + 2 * DR;
-constexpr float kDirSpread = 0.2f;
+constexpr int DR = 3, AD_TX = 64, AD_TY = 16, AD_NT = 256;
+constexpr int AD_SX = AD_TX + 2 * DR, AD_SY = AD_TY + 2 * DR, AD_SN = (AD_SX * AD_SY + AD_NT - 1) / AD_NT;
 
 // the k-th tap of sender (x, y) with unit direction (ex, ey), as derivative_pixel forms it
 __device__ __forceinline__ void deriv_tap(int k, float x, float y, float ex, float ey, float& tx, float& ty, float& wt)
@@ -340,100 +339,92 @@ __device__ __forceinline__ void deriv_tap(int k, float x, float y, float ex, flo
         wt = -1.0f;
     }
 }
+
+// one corner share: into the tile's fixed-point sum if the corner is a texel of the tile
+__device__ __forceinline__ void add_share(unsigned long long* acc, unsigned* bad, int cx, int cy, int X0, int Y0,
+                                          float share, double scale)
+{
+    const int lx = cx - X0, ly = cy - Y0;
+    if (lx < 0 || lx >= AD_TX || ly < 0 || ly >= AD_TY) return;
+    const int j = ly * AD_TX + lx;
+    if (!isfinite(share)) {
+        atomicOr(&bad[j], 1u);
+        return;
+    }
+    const long long q = llrint(static_cast<double>(share) * scale);
+    atomicAdd(&acc[j], static_cast<unsigned long long>(q));
+}
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_adj_derivatives(float* __restrict__ gref, float* __restrict__ gow,
-                                                         const float* __restrict__ gderiv,
-                                                         const float2* __restrict__ tvf, int H, int W)
+__global__ void __launch_bounds__(AD_NT) k_adj_derivatives(float* __restrict__ gref, float* __restrict__ gow,
+                                                           const float* __restrict__ gderiv,
+                                                           const float2* __restrict__ tvf, int H, int W)
 {
-    __shared__ float s_g[DLY][DLX], s_ex[DLY][DLX], s_ey[DLY][DLX];
-    const int X0 = blockIdx.x * DTX, Y0 = blockIdx.y * DTY;
-    const int tid = threadIdx.y * DTX + threadIdx.x;
-    for (int t = tid; t < DLX * DLY; t += DTX * DTY) {
-        const int lx = t % DLX, ly = t / DLX, sx = X0 - DR + lx, sy = Y0 - DR + ly;
-        float g = 0.0f, ex = 0.0f, ey = 0.0f;
-        if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
-            const int s = sy * W + sx;
-            g = gderiv[2 * s] / 24.0f;
-            if (g != 0.0f) {
-                const float2 vec = tvf[s];
-                const float nrm = hypotf(vec.x, vec.y);
-                ex = vec.x / nrm;
-                ey = vec.y / nrm;
-            }
-        }
-        s_g[ly][lx] = g;
-        s_ex[ly][lx] = ex;
-        s_ey[ly][lx] = ey;
+    __shared__ unsigned long long s_acc[AD_TX * AD_TY];
+    __shared__ unsigned s_bad[AD_TX * AD_TY];
+    __shared__ float s_wmax[AD_NT / 64];
+    const int X0 = blockIdx.x * AD_TX, Y0 = blockIdx.y * AD_TY;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < AD_TX * AD_TY; j += AD_NT) {
+        s_acc[j] = 0ull;
+        s_bad[j] = 0u;
     }
-    __syncthreads();
-    const int X = X0 + threadIdx.x, Y = Y0 + threadIdx.y;
-    if (X >= W || Y >= H) return;
-    const int idx = Y * W + X;
-    const int lx0 = threadIdx.x + DR, ly0 = threadIdx.y + DR;
-    // spread of the directions over the active senders of the 7 x 7 window
-    float mnx = 2.0f, mxx = -2.0f, mny = 2.0f, mxy = -2.0f;
-    bool finite = true;
-    for (int dy = -DR; dy <= DR; ++dy)
-        for (int dx = -DR; dx <= DR; ++dx) {
-            if (s_g[ly0 + dy][lx0 + dx] == 0.0f) continue;
-            const float ex = s_ex[ly0 + dy][lx0 + dx], ey = s_ey[ly0 + dy][lx0 + dx];
-            finite = finite && (ex == ex) && (ey == ey);
-            mnx = fminf(mnx, ex);
-            mxx = fmaxf(mxx, ex);
-            mny = fminf(mny, ey);
-            mxy = fmaxf(mxy, ey);
+    // the senders' giu (g(Iu) / 24) and the largest finite |giu|
+    float g[AD_SN];
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < AD_SN; ++i) {
+        const int e = tid + AD_NT * i;
+        const int sx = X0 - DR + e % AD_SX, sy = Y0 - DR + e / AD_SX;
+        g[i] = 0.0f;
+        if (e < AD_SX * AD_SY && sx >= 0 && sx < W && sy >= 0 && sy < H) {
+            g[i] = gderiv[2 * (sy * W + sx)] / 24.0f;
+            if (isfinite(g[i])) m = fmaxf(m, fabsf(g[i]));
         }
-    const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny);
-    const bool interior = X >= DR && X < W - DR && Y >= DR && Y < H - DR;
-    float acc = 0.0f;
-    if (interior && finite && hx <= kDirSpread && hy <= kDirSpread) {
-        const float cx = 0.5f * (mxx + mnx), cy = 0.5f * (mxy + mny);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((tid & 63) == 0) s_wmax[tid >> 6] = m;
+    __syncthreads();
+    m = s_wmax[0];
+#pragma unroll
+    for (int w = 1; w < AD_NT / 64; ++w) m = fmaxf(m, s_wmax[w]);
+    // |share| <= 8 M < 2^(e + 3) for M < 2^e: S = 54 - (e + 3)
+    int ex2 = 0;
+    frexpf(m, &ex2);
+    const double scale = m > 0.0f ? ldexp(1.0, 51 - ex2) : 1.0;
+#pragma unroll
+    for (int i = 0; i < AD_SN; ++i) {
+        const int e = tid + AD_NT * i;
+        if (g[i] == 0.0f) continue;  // also every sender outside the image
+        const int sx = X0 - DR + e % AD_SX, sy = Y0 - DR + e / AD_SX;
+        const float2 vec = tvf[sy * W + sx];
+        const float nrm = hypotf(vec.x, vec.y);
+        const float ex = vec.x / nrm, ey = vec.y / nrm;
         for (int k = -2; k <= 2; ++k) {
             if (k == 0) continue;
-            const float kk = static_cast<float>(k), ak = fabsf(kk);
-            // s in [r - 1 - k c - |k| h, r + 1 - k c + |k| h), widened by 0.01 for rounding
-            int xlo = static_cast<int>(ceilf(X - 1.0f - kk * cx - ak * hx - 0.01f));
-            int xhi = static_cast<int>(ceilf(X + 1.0f - kk * cx + ak * hx + 0.01f)) - 1;
-            int ylo = static_cast<int>(ceilf(Y - 1.0f - kk * cy - ak * hy - 0.01f));
-            int yhi = static_cast<int>(ceilf(Y + 1.0f - kk * cy + ak * hy + 0.01f)) - 1;
-            xlo = xlo < X - DR ? X - DR : xlo;
-            xhi = xhi > X + DR ? X + DR : xhi;
-            ylo = ylo < Y - DR ? Y - DR : ylo;
-            yhi = yhi > Y + DR ? Y + DR : yhi;
-            for (int sy = ylo; sy <= yhi; ++sy)
-                for (int sx = xlo; sx <= xhi; ++sx) {
-                    const int lx = sx - X0 + DR, ly = sy - Y0 + DR;
-                    const float g = s_g[ly][lx];
-                    if (g == 0.0f) continue;
-                    float tx, ty, wt;
-                    deriv_tap(k, static_cast<float>(sx), static_cast<float>(sy), s_ex[ly][lx], s_ey[ly][lx], tx, ty,
-                              wt);
-                    acc += tap_share(idx, H, W, tx, ty, g * wt);
-                }
-        }
-    } else {
-        for (int dy = -DR; dy <= DR; ++dy) {
-            const int sy = Y + dy;
-            if (sy < 0 || sy >= H) continue;
-            for (int dx = -DR; dx <= DR; ++dx) {
-                const int sx = X + dx;
-                if (sx < 0 || sx >= W) continue;
-                const float g = s_g[ly0 + dy][lx0 + dx];
-                if (g == 0.0f) continue;
-                const float ex = s_ex[ly0 + dy][lx0 + dx], ey = s_ey[ly0 + dy][lx0 + dx];
-                for (int k = -2; k <= 2; ++k) {
-                    if (k == 0) continue;
-                    float tx, ty, wt;
-                    deriv_tap(k, static_cast<float>(sx), static_cast<float>(sy), ex, ey, tx, ty, wt);
-                    acc += tap_share(idx, H, W, tx, ty, g * wt);
-                }
-            }
+            float tx, ty, wt;
+            deriv_tap(k, static_cast<float>(sx), static_cast<float>(sy), ex, ey, tx, ty, wt);
+            if (!(tx == tx) || !(ty == ty)) continue;
+            const float v = g[i] * wt;
+            const TapXY t = make_tap_xy(tx, ty, H, W);
+            add_share(s_acc, s_bad, t.x0, t.y0, X0, Y0, v * (1.0f - t.ax) * (1.0f - t.ay), scale);
+            add_share(s_acc, s_bad, t.x1, t.y0, X0, Y0, v * t.ax * (1.0f - t.ay), scale);
+            add_share(s_acc, s_bad, t.x0, t.y1, X0, Y0, v * (1.0f - t.ax) * t.ay, scale);
+            add_share(s_acc, s_bad, t.x1, t.y1, X0, Y0, v * t.ax * t.ay, scale);
         }
     }
-    const float giz = gderiv[2 * idx + 1];
-    gow[idx] += acc + giz;
-    gref[idx] += acc - giz;
+    __syncthreads();
+    for (int j = tid; j < AD_TX * AD_TY; j += AD_NT) {
+        const int X = X0 + j % AD_TX, Y = Y0 + j / AD_TX;
+        if (X >= W || Y >= H) continue;
+        const int idx = Y * W + X;
+        const float acc = s_bad[j] ? __builtin_nanf("")
+                                   : static_cast<float>(static_cast<double>(static_cast<long long>(s_acc[j])) / scale);
+        const float giz = gderiv[2 * idx + 1];
+        gow[idx] += acc + giz;
+        gref[idx] += acc - giz;
+    }
 }
 
 // derivative of a bilinear sample with respect to its coordinates
@@ -936,7 +927,8 @@ int vfs_adj_derivatives(void* stream, float* gref, float* gow, const float* gder
     VFS_CHECK_PTR(gow);
     VFS_CHECK_PTR(gderiv);
     VFS_CHECK_PTR(tvf);
-    hipLaunchKernelGGL(k_adj_derivatives, grid2d(H, W), block2d(), 0, S(stream), gref, gow, gderiv,
+    const dim3 grid((W + AD_TX - 1) / AD_TX, (H + AD_TY - 1) / AD_TY, 1);
+    hipLaunchKernelGGL(k_adj_derivatives, grid, dim3(AD_NT), 0, S(stream), gref, gow, gderiv,
                        reinterpret_cast<const float2*>(tvf), H, W);
     VFS_LAUNCH_STATUS();
 }

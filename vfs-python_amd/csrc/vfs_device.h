@@ -38,10 +38,16 @@ struct Tap {
     float ax, ay;
 };
 
+// The same tap set as columns x0, x1 and rows y0, y1 of the image.
+struct TapXY {
+    int x0, x1, y0, y1;
+    float ax, ay;
+};
+
 // Bilinear tap set at texel coordinate (x, y).  Non-finite coordinates keep
 // their NaN/inf in the weights so that the sample is non-finite, while the
 // indices stay in range.
-__device__ __forceinline__ Tap make_tap(float x, float y, int H, int W)
+__device__ __forceinline__ TapXY make_tap_xy(float x, float y, int H, int W)
 {
     const float xc = fminf(fmaxf(x, -1.0e7f), 1.0e7f);
     const float yc = fminf(fmaxf(y, -1.0e7f), 1.0e7f);
@@ -49,15 +55,26 @@ __device__ __forceinline__ Tap make_tap(float x, float y, int H, int W)
     const float fy = floorf(yc);
     const int ix = static_cast<int>(fx);
     const int iy = static_cast<int>(fy);
-    const int x0 = mirror(ix, W), x1 = mirror(ix + 1, W);
-    const int y0 = mirror(iy, H), y1 = mirror(iy + 1, H);
-    Tap t;
-    t.i00 = y0 * W + x0;
-    t.i10 = y0 * W + x1;
-    t.i01 = y1 * W + x0;
-    t.i11 = y1 * W + x1;
+    TapXY t;
+    t.x0 = mirror(ix, W);
+    t.x1 = mirror(ix + 1, W);
+    t.y0 = mirror(iy, H);
+    t.y1 = mirror(iy + 1, H);
     t.ax = x - fx;
     t.ay = y - fy;
+    return t;
+}
+
+__device__ __forceinline__ Tap make_tap(float x, float y, int H, int W)
+{
+    const TapXY u = make_tap_xy(x, y, H, W);
+    Tap t;
+    t.i00 = u.y0 * W + u.x0;
+    t.i10 = u.y0 * W + u.x1;
+    t.i01 = u.y1 * W + u.x0;
+    t.i11 = u.y1 * W + u.x1;
+    t.ax = u.ax;
+    t.ay = u.ay;
     return t;
 }
 
