@@ -264,6 +264,24 @@ __global__ void __launch_bounds__(256) k_add(float* __restrict__ out, const floa
     if (i < n) out[i] += in[i];
 }
 
+// A warp's reverse sweep starts from gubB := gub (the adjoint of u_bar at the warp's
+// end) and zero v_bar, p, q adjoints, derivative and warped-image accumulators.
+__global__ void __launch_bounds__(256) k_warp_reset(float* __restrict__ gubB, const float* __restrict__ gub,
+                                                    float2* __restrict__ gvbB, float2* __restrict__ gp,
+                                                    float4* __restrict__ gq, float2* __restrict__ gderiv,
+                                                    float* __restrict__ gow, int64_t n)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const float2 z2 = make_float2(0.0f, 0.0f);
+    gubB[i] = gub[i];
+    gvbB[i] = z2;
+    gp[i] = z2;
+    gq[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    gderiv[i] = z2;
+    gow[i] = 0.0f;
+}
+
 // One step of the fused reverse sweep over a tile of 32 x 8R pixels (256 threads,
 // R rows each).  With S_i the state before iteration i (S_0 the warp start),
 // launch F(i) runs
@@ -1322,11 +1340,13 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         VFS_TRY(vfs_adj_clamp(stream, gu, A.guf, A.pos, A.gtv, gf, u_th, ws.u, lv->tv, lv->mask, H, W,
                               prm->limit_range, prm->median_k));
         VFS_TRY(vfs_adj_median(stream, A.guth, A.guf, A.pos, H, W, prm->median_k));
-        VFS_TRY(vfs_copy(stream, A.gubB, gub, n));
-        VFS_TRY(vfs_fill(stream, A.gvbB, 0.0f, 2 * n));
-        VFS_TRY(vfs_fill(stream, A.gp, 0.0f, 2 * n));
-        VFS_TRY(vfs_fill(stream, A.gq, 0.0f, 4 * n));
-        VFS_TRY(vfs_fill(stream, A.gderiv, 0.0f, 2 * n));
+        hipLaunchKernelGGL(k_warp_reset, dim3(nb1(n)), dim3(256), 0, s, A.gubB, gub, reinterpret_cast<float2*>(A.gvbB),
+                           reinterpret_cast<float2*>(A.gp), reinterpret_cast<float4*>(A.gq),
+                           reinterpret_cast<float2*>(A.gderiv), A.gow, n);
+        if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+        // the adjoint of the warp-start u after the sweep (the blocked sweep may end in its
+        // second buffer set, which the tail then reads in place)
+        float* fin_gu = A.guth;
         if (nck > 0 && fused) {
             // fused sweep: head PL(n-1), F(i) for i = n-1 .. 1, tail D(0) + finish
             int cs = nck - 1;
@@ -1457,17 +1477,19 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                     scur = 1 - scur;
                     i -= kc;
                 }
-                if (scur == 1) {
-                    VFS_TRY(vfs_copy(stream, A.guth, A.gu2, n));
-                    VFS_TRY(vfs_copy(stream, A.gubB, A.gub2, n));
-                    VFS_TRY(vfs_copy(stream, A.gvbB, A.gvb2, 2 * n));
-                }
+                fin_gu = gus[scur];
+                fa.gu = gus[scur];
+                fa.gubB = gubs[scur];
+                fa.gvbB = reinterpret_cast<float2*>(gvbs[scur]);
             }
             const State s0 = state_at(ws.ckpt0, n);
             set_states(s0, s0);
             fa.st = dual_steps(sigmas[0], prm->alpha0, prm->alpha1, prm->eta_p, prm->eta_q);
             set_bufs();
             VFS_TRY(launch_fused_rows(rows, 2, fblocks, s, fa));
+            fa.gu = A.guth;
+            fa.gubB = A.gubB;
+            fa.gvbB = reinterpret_cast<float2*>(A.gvbB);
         } else if (nck > 0) {
             // one kernel per step (primal^T, threshold^T, dual^T), messages through memory
             int has_msgs = 0;
@@ -1511,9 +1533,8 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
             // adjoint of u_bar at the warp start; v_bar, p, q start at zero
             VFS_TRY(vfs_adj_pd_finish(stream, gub, A.gvb_fin, A.gubB, A.gvbB, A.um, A.vm, lv->mask, H, W));
         }
-        hipLaunchKernelGGL(k_add, dim3(nb1(n)), dim3(256), 0, s, gu, A.guth, n);
+        hipLaunchKernelGGL(k_add, dim3(nb1(n)), dim3(256), 0, s, gu, fin_gu, n);
         if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
-        VFS_TRY(vfs_fill(stream, A.gow, 0.0f, n));
         VFS_TRY(vfs_adj_derivatives(stream, gref, A.gow, A.gderiv, lv->tvf, H, W));
         VFS_TRY(vfs_adj_warp(stream, gf, A.gow, A.gtv, ws.flow, lv->other, lv->tvf, H, W));
     }
