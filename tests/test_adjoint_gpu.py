@@ -290,8 +290,9 @@ def test_gradients_do_not_depend_on_checkpoints_or_kernel_choice(T):
 @pytest.mark.parametrize("shape,levels,ratio", [((17, 23), 2, 2.0), ((64, 64), 2, 2.0), ((44, 52), 4, 1.2)])
 def test_gradients_are_bitwise_reproducible(T, shape, levels, ratio):
     """Two gradients of the same pair, back to back on fresh objects, are the same bits:
-    no reverse kernel adds with atomics (the transposes of the derivative taps, tensor,
-    Gaussian, pyramid and upsampling are gathers in a fixed order)."""
+    no reverse kernel adds floats with atomics (the transposes of the tensor, Gaussian,
+    pyramid and upsampling are gathers in a fixed order; the derivative taps' shares are
+    summed in int64 fixed point, whose sum does not depend on the order)."""
     import synthetic as S
     import vfs_adjoint as A
     import vfs_params as P
@@ -313,6 +314,37 @@ def test_gradients_are_bitwise_reproducible(T, shape, levels, ratio):
     assert l0 == l1
     assert np.isfinite([g0[k] for k in ("Lambda", "alpha0", "alpha1", "beta", "gamma")]).all()
     _assert_same_bits(g0, g1)
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_concurrent_gradients_equal_sequential_ones(T, shared):
+    """vfs_adjoint.gradients runs three pairs on two streams (two host threads): the
+    losses and gradients equal forward(); backward() run one pair after another, bit for
+    bit, with each pair's own workspace and with one shared workspace per stream."""
+    import synthetic as S
+    import vfs_adjoint as A
+    import vfs_params as P
+    items, seq = [], []
+    for seed, (hh, ww) in zip((21, 22, 23), ((40, 48), (33, 29), (48, 40))):
+        pr = S.make_pair(hh, ww, seed, t=(0.2, 0.0, 0.02), rot_deg=1.0, n_levels=2)
+        tvf, cvf = O.get_vector_fields(O.get_grid_zero((hh, ww)), pr["cam"], pr["cam"], pr["R"], pr["t"], 1)
+        tvf, cvf = S.sanitize_field(tvf), cvf.astype(np.float32)
+        prm = P.vfs_params(pr["img_ref"])
+        prm.nLevel, prm.fScale, prm.nWarpIters, prm.nSolverIters = 2, 2.0, 3, 9
+        for k, v in THETA.items():
+            setattr(prm, k, v)
+        ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+        make = lambda: A.VFSDepthAdjoint(prm, pr["mask"], tvf, cvf, pr["cam"], pr["t"], pr["z_gt"],  # noqa: E731
+                                         checkpoint=4, workspace={} if shared else None)
+        a = make()
+        seq.append((a.forward(ref0, pr["img_other"]), a.backward()))
+        items.append((make(), ref0, pr["img_other"]))
+    ws = [{}, {}] if shared else None
+    for _ in range(2):
+        got = A.gradients(items, streams=2, workspaces=ws)
+        for (l0, g0), (l1, g1) in zip(seq, got):
+            assert l0 == l1
+            _assert_same_bits(g0, g1)
 
 
 def test_reverse_probe_times_the_blocked_launches_and_leaves_gradients_unchanged(T):

@@ -6,8 +6,9 @@ over the mask pixels whose ground-truth disparity is at least ``min_disparity``
 pixels (near the epipoles the disparity vanishes and triangulation is
 ill-conditioned: those pixels would dominate a mean of |log z - log z_gt|).  A
 step takes the mean loss and gradient over a chunk of pairs and updates the
-logarithms of the five parameters with Adam (they stay positive).  Pairs are
-solved one after another on one stream and share one reverse workspace; at most
+logarithms of the five parameters with Adam (they stay positive).  The pairs of a
+step run on ``streams`` HIP streams at once (``vfs_adjoint.gradients``), the pairs of
+one stream one after another sharing that stream's reverse workspace; at most
 ``resident`` pairs keep their adjoint buffers on the GPU (least recently used
 first out), so a step over many pairs streams them through in chunks.
 
@@ -133,9 +134,10 @@ class ParameterFit:
     """Adam on log(theta) with the mean depth-loss gradient over chunks of pairs."""
 
     def __init__(self, pairs, theta0=START, lr=0.05, betas=(0.9, 0.999), eps=1e-8, reverse_warps=1,
-                 min_disparity=MIN_DISPARITY, resident=32):
+                 min_disparity=MIN_DISPARITY, resident=32, streams=2):
         self.pairs = pairs
-        self.ws = {}
+        self.streams = max(1, int(streams))
+        self.ws = [{} for _ in range(self.streams)]  # one shared reverse workspace per stream
         self.theta0 = dict(theta0)
         self.reverse_warps = int(reverse_warps)
         self.min_disparity = float(min_disparity)
@@ -160,7 +162,7 @@ class ParameterFit:
         pr = self.pairs[i]
         a = A.VFSDepthAdjoint(make_param(pr["img_ref"], self.theta0), pr["mask"], pr["tvf"], pr["cvf"], pr["cam"],
                               pr["t"], loss_target(pr, self.min_disparity), checkpoint=SCHEDULE["checkpoint"],
-                              workspace=self.ws, reverse_warps=self.reverse_warps)
+                              workspace=self.ws[0], reverse_warps=self.reverse_warps)
         ref = torch.as_tensor(pr["img_ref"], device="cuda").to(torch.float32) / 256.0
         other = torch.as_tensor(pr["img_other"], device="cuda")
         self._adj[i] = (a, ref, other)
@@ -172,12 +174,16 @@ class ParameterFit:
         """Mean loss and d(mean loss)/d(log theta) over the pairs idx."""
         th = np.exp(self.phi)
         tot, g = 0.0, np.zeros(5)
-        for i in idx:
-            a, ref, other = self.adjoint(i)
-            a.set_params(th)
-            tot += a.forward(ref, other)
-            gr = a.backward()
-            g += np.array([gr[k] for k in PARAMS])
+        # chunks of at most `resident` pairs, each chunk's pairs on `streams` streams
+        for c in range(0, len(idx), self.resident):
+            items = []
+            for i in idx[c:c + self.resident]:
+                a, ref, other = self.adjoint(i)
+                a.set_params(th)
+                items.append((a, ref, other))
+            for loss, gr in A.gradients(items, streams=self.streams, workspaces=self.ws):
+                tot += loss
+                g += np.array([gr[k] for k in PARAMS])
         n = len(idx)
         return tot / n, (g / n) * th
 

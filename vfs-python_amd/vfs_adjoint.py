@@ -26,6 +26,7 @@ of saves, plus one workspace of (43 + 10 checkpoint) floats per pixel of level 0
 
 import copy
 import ctypes
+import threading
 
 import numpy as np
 import torch
@@ -185,18 +186,25 @@ class VFSDepthAdjoint:
             b["gref"].append(_z(sz))
             b["gpar"].append(_z(sz, 5))
         ws = max(L.vfs_adj_workspace_bytes(sz[0], sz[1], self.checkpoint) for sz in sizes)
-        if self._shared_ws is not None:
-            cur = self._shared_ws.get("ws")
-            if cur is None or cur.numel() < ws:
-                self._shared_ws["ws"] = torch.empty(ws, dtype=torch.uint8, device="cuda")
-            b["ws"] = None
-        else:
-            b["ws"] = torch.empty(ws, dtype=torch.uint8, device="cuda")
+        b["ws_bytes"] = ws
+        b["ws"] = None if self._shared_ws is not None else torch.empty(ws, dtype=torch.uint8, device="cuda")
         self._b = b
         return b
 
     def _ws(self):
-        return self._shared_ws["ws"] if self._shared_ws is not None else self._b["ws"]
+        """The reverse workspace: this pair's own, or the shared one (grown to this pair's size)."""
+        if self._shared_ws is None:
+            return self._b["ws"]
+        cur = self._shared_ws.get("ws")
+        if cur is None or cur.numel() < self._b["ws_bytes"]:
+            cur = self._shared_ws["ws"] = torch.empty(self._b["ws_bytes"], dtype=torch.uint8, device="cuda")
+        return cur
+
+    def set_workspace(self, workspace):
+        """Share the reverse workspace dict `workspace` from now on (None: a workspace of its own)."""
+        self._shared_ws = workspace
+        if self._b is not None and workspace is None and self._b["ws"] is None:
+            self._b["ws"] = torch.empty(self._b["ws_bytes"], dtype=torch.uint8, device="cuda")
 
     def _reduce(self, x, channels):
         """Device (channels,) float64 sums of an interleaved float32 array."""
@@ -372,3 +380,60 @@ class DepthLoss(torch.autograd.Function):
         gtheta = torch.tensor([g[n] * gl for n in PARAMS], dtype=dt, device=dev)
         rdev, rdt = ctx.ref0_meta
         return (g["ref0"] * gl).to(device=rdev, dtype=rdt), gtheta, None, None
+
+
+_streams = []
+
+
+def gradients(items, streams=2, workspaces=None):
+    """Loss and gradients of several pairs, run concurrently on `streams` HIP streams.
+
+    items: list of (adj, ref0, img_other), adj a VFSDepthAdjoint.  Pair i runs forward()
+    then backward() on stream i mod streams, one host thread per stream, so that the
+    launches of different pairs overlap on the GPU (their coarse levels and per-warp
+    kernels do not fill it alone).  A pair's arithmetic does not depend on the stream:
+    the gradients are the same bits as forward(); backward() one pair after another.
+    Pairs on different streams must not share a reverse workspace: with `workspaces` (a
+    list of `streams` workspace dicts) each pair uses its stream's dict (set_workspace);
+    without it every pair keeps the workspace it was built with, which must then be its
+    own or shared only within one stream.  Returns [(loss, gradient dict)] in the order
+    of items; the streams are synchronised on return.  A worker's exception is re-raised."""
+    streams = max(1, int(streams))
+    if workspaces is not None and len(workspaces) != streams:
+        raise ValueError("gradients: one workspace dict per stream")
+    while len(_streams) < streams:
+        _streams.append(torch.cuda.Stream())
+    out = [None] * len(items)
+    errors = []
+
+    def worker(k):
+        try:
+            st = _streams[k]
+            st.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(st):
+                for i in range(k, len(items), streams):
+                    adj, ref0, other = items[i]
+                    if workspaces is not None:
+                        adj.set_workspace(workspaces[k])
+                    loss = adj.forward(ref0, other)
+                    out[i] = (loss, adj.backward())
+                st.synchronize()
+        except BaseException as e:  # re-raised by the caller's thread
+            errors.append(e)
+
+    if streams == 1:
+        worker(0)
+    else:
+        threads = [threading.Thread(target=worker, args=(k,)) for k in range(streams)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    if errors:
+        raise errors[0]
+    cur = torch.cuda.current_stream()
+    for k in range(streams):
+        cur.wait_stream(_streams[k])
+    for _, g in out:
+        g["ref0"].record_stream(cur)
+    return out
