@@ -67,26 +67,19 @@ __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2
                                            bool fwd_y, float a, float b, float c, float2 pc, float4 qc, float scale_p,
                                            float scale_q, float2& pn, float2& tpn, float4& qn)
 {
+    // Forward differences, negated for a backward neighbour: b - a is exactly -(a - b), so
+    // selecting the sign gives the same bits as the two differences written out, without
+    // a divergent branch.
+    const float dux = uX - uc, dvxx = vX.x - vc.x, dvxy = vX.y - vc.y;
+    const float duy = uY - uc, dvyy = vY.y - vc.y, dvyx = vY.x - vc.x;
     float2 gU;
     float4 gV;
-    if (fwd_x) {
-        gU.x = uX - uc - vc.x;
-        gV.x = vX.x - vc.x;
-        gV.w = vX.y - vc.y;
-    } else {
-        gU.x = uc - uX - vc.x;
-        gV.x = vc.x - vX.x;
-        gV.w = vc.y - vX.y;
-    }
-    if (fwd_y) {
-        gU.y = uY - uc - vc.y;
-        gV.y = vY.y - vc.y;
-        gV.z = vY.x - vc.x;
-    } else {
-        gU.y = uc - uY - vc.y;
-        gV.y = vc.y - vY.y;
-        gV.z = vc.x - vY.x;
-    }
+    gU.x = (fwd_x ? dux : -dux) - vc.x;
+    gV.x = fwd_x ? dvxx : -dvxx;
+    gV.w = fwd_x ? dvxy : -dvxy;
+    gU.y = (fwd_y ? duy : -duy) - vc.y;
+    gV.y = fwd_y ? dvyy : -dvyy;
+    gV.z = fwd_y ? dvyx : -dvyx;
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
     const float2 up = make_float2(fmaf(scale_p, tu.x, pc.x), fmaf(scale_p, tu.y, pc.y));
     const float rp = unit_ball_scale(sq_norm2(up.x, up.y));
@@ -122,13 +115,15 @@ __device__ __forceinline__ float l1_pixel(float2 tc, float tpWx, float tpNy, boo
     const float uTauGtp = ubc + tauEtaU * (gx + gy);
     const float rho = dIu * (uTauGtp - uc) + dIz;
     const float bound = lambda * tauEtaU * dIu * dIu;
-    float du = 0.0f;
-    if ((rho <= bound) && (rho >= -bound))
-        du = (uTauGtp - uc) - rho * rIu;
-    else if (rho < -bound)
-        du = (uTauGtp - uc) + lambda * tauEtaU * dIu;
-    else if (rho > bound)
-        du = (uTauGtp - uc) - lambda * tauEtaU * dIu;
+    // the three branches of the prox, each the same expression as written per branch,
+    // selected without a divergent branch (none of them for a NaN rho: du = 0)
+    const float du_in = (uTauGtp - uc) - rho * rIu;
+    const float du_lo = (uTauGtp - uc) + lambda * tauEtaU * dIu;
+    const float du_hi = (uTauGtp - uc) - lambda * tauEtaU * dIu;
+    const float du = ((rho <= bound) && (rho >= -bound)) ? du_in
+                     : (rho < -bound)                    ? du_lo
+                     : (rho > bound)                     ? du_hi
+                                                         : 0.0f;
     return uc + du;
 }
 
