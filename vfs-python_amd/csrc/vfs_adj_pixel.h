@@ -216,28 +216,27 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
     const float rho = dIu * (ut - uc) + dIz;
     const float bound = lambda * te * dIu * dIu;
     const float d0 = ut - uc;
-    float g_d0 = 0.0f, g_dIu = 0.0f, g_dIz = 0.0f, g_te = 0.0f, g_lam = 0.0f;
-    if ((rho <= bound) && (rho >= -bound)) {
-        if (fabsf(dIu) < kEps) {
-            g_d0 = gout;
-        } else {  // du = d0 - rho r, r = 1 / dIu
-            const float r = recip_grad(dIu, kEps);
-            const float g_rho = -gout * r;
-            g_d0 = gout + g_rho * dIu;
-            g_dIu = gout * rho * r * r + g_rho * d0;
-            g_dIz = g_rho;
-        }
-    } else if (rho < -bound) {  // du = d0 + lambda te dIu
-        g_d0 = gout;
-        g_lam = gout * te * dIu;
-        g_te = gout * lambda * dIu;
-        g_dIu = gout * lambda * te;
-    } else if (rho > bound) {  // du = d0 - lambda te dIu
-        g_d0 = gout;
-        g_lam = -gout * te * dIu;
-        g_te = -gout * lambda * dIu;
-        g_dIu = -gout * lambda * te;
-    }
+    // The branch the forward took, with each branch's expressions as written for it,
+    // formed everywhere and selected (no divergent branch): inside the band with
+    // |dIu| >= eps, du = d0 - rho r (r = 1 / dIu); inside with |dIu| < eps, du = d0;
+    // below the band du = d0 + lambda te dIu, above it du = d0 - lambda te dIu; none of
+    // them for a NaN rho (du = 0).
+    const bool in_band = (rho <= bound) && (rho >= -bound);
+    const bool lo = !in_band && (rho < -bound);
+    const bool hi = !in_band && !lo && (rho > bound);
+    const bool flat = fabsf(dIu) < kEps;
+    const float r = recip_grad(dIu, kEps);
+    const float g_rho = -gout * r;
+    const float gi_d0 = gout + g_rho * dIu;
+    const float gi_dIu = gout * rho * r * r + g_rho * d0;
+    const float glo_lam = gout * te * dIu, glo_te = gout * lambda * dIu, glo_dIu = gout * lambda * te;
+    const float ghi_lam = -gout * te * dIu, ghi_te = -gout * lambda * dIu, ghi_dIu = -gout * lambda * te;
+    const bool band_grad = in_band && !flat;
+    const float g_d0 = band_grad ? gi_d0 : ((in_band || lo || hi) ? gout : 0.0f);
+    const float g_dIu = band_grad ? gi_dIu : (lo ? glo_dIu : (hi ? ghi_dIu : 0.0f));
+    const float g_dIz = band_grad ? g_rho : 0.0f;
+    float g_te = lo ? glo_te : (hi ? ghi_te : 0.0f);
+    const float g_lam = lo ? glo_lam : (hi ? ghi_lam : 0.0f);
     // out = uc + du, d0 = ut - uc, ut = ubc + te (gx + gy)
     const float g_ut = g_d0;
     gubB += g_ut;
@@ -348,26 +347,18 @@ __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float
                                                float& gpa0, float& gpa1)
 {
     const float sx = fx ? 1.0f : -1.0f, sy = fy ? 1.0f : -1.0f;
+    // the forward differences of dual_pixel, negated for a backward neighbour (b - a is
+    // exactly -(a - b)): the same bits without a divergent branch
+    const float dux = uX - uc, dvxx = vX.x - vc.x, dvxy = vX.y - vc.y;
+    const float duy = uY - uc, dvyy = vY.y - vc.y, dvyx = vY.x - vc.x;
     float2 gU;
     float4 gV;
-    if (fx) {
-        gU.x = uX - uc - vc.x;
-        gV.x = vX.x - vc.x;
-        gV.w = vX.y - vc.y;
-    } else {
-        gU.x = uc - uX - vc.x;
-        gV.x = vc.x - vX.x;
-        gV.w = vc.y - vX.y;
-    }
-    if (fy) {
-        gU.y = uY - uc - vc.y;
-        gV.y = vY.y - vc.y;
-        gV.z = vY.x - vc.x;
-    } else {
-        gU.y = uc - uY - vc.y;
-        gV.y = vc.y - vY.y;
-        gV.z = vc.x - vY.x;
-    }
+    gU.x = (fx ? dux : -dux) - vc.x;
+    gV.x = fx ? dvxx : -dvxx;
+    gV.w = fx ? dvxy : -dvxy;
+    gU.y = (fy ? duy : -duy) - vc.y;
+    gV.y = fy ? dvyy : -dvyy;
+    gV.z = fy ? dvyx : -dvyx;
     const float sp = st.sp, sq = st.sq;
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
     const float2 up = make_float2(fmaf(sp, tu.x, pc.x), fmaf(sp, tu.y, pc.y));
