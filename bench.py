@@ -175,6 +175,9 @@ def fit_leg():
         held_out_median_rel_depth_err={"start": round(r["held_out_median_rel_depth_err_start"], 5),
                                        "fitted": round(r["held_out_median_rel_depth_err_fitted"], 5)},
         train_loss=[round(x, 5) for x in r["train_loss"]],
+        best_by_train_loss={"step": r["best_step"], "theta": {k: round(v, 5) for k, v in r["best_theta"].items()},
+                            "held_out_loss": round(r["held_out_loss_best"], 5),
+                            "held_out_median_rel_depth_err": round(r["held_out_median_rel_depth_err_best"], 5)},
         # the fit's kernels are the gradient leg's (forward blocked launches, reverse sweep);
         # their roofline and CPU baseline are reported there
         roofline=None, cpu_baseline=None)

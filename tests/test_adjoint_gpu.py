@@ -45,6 +45,39 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
+@pytest.mark.parametrize("shape", [(70, 150), (33, 64), (5, 3)])
+def test_derivative_taps_transpose_matches_autograd(T, shape):
+    """vfs_adj_derivatives (the taps' shares summed in int64 fixed point per block) against
+    torch autograd of the float32 restatement of computeDerivatives
+    (oracle/vfs_torch_ref.compute_derivatives) on random images, directions and output
+    adjoints, over several 64 x 16 blocks, the image border (mirrored taps) and an image
+    smaller than a tap's reach.  The tap coordinates and bilinear weights are the same
+    float32 values on both sides (up to the last bit of the normalised direction), so the
+    two differ by the rounding of the shares' products and of autograd's float32 sums:
+    every entry within 2e-6 of the largest."""
+    import vfs_hip as H
+    hh, ww = shape
+    rng = np.random.default_rng(hh * 1000 + ww)
+    ref = rng.random((hh, ww), dtype=np.float32)
+    ow = rng.random((hh, ww), dtype=np.float32)
+    ang = rng.uniform(0.0, 2.0 * np.pi, (hh, ww))
+    mag = rng.uniform(0.5, 2.0, (hh, ww, 1))
+    tvf = (np.stack([np.cos(ang), np.sin(ang)], -1) * mag).astype(np.float32)
+    gd = rng.normal(size=(hh, ww, 2)).astype(np.float32)
+    r = torch.tensor(ref, requires_grad=True)
+    o = torch.tensor(ow, requires_grad=True)
+    (R.compute_derivatives(o, r, torch.tensor(tvf)) * torch.tensor(gd)).sum().backward()
+    gref = T.zeros((hh, ww), device="cuda")
+    gow = T.zeros((hh, ww), device="cuda")
+    dgd, dtvf = T.tensor(gd, device="cuda"), T.tensor(tvf, device="cuda")
+    H.call("vfs_adj_derivatives", H.stream(), H.ptr(gref), H.ptr(gow), H.ptr(dgd), H.ptr(dtvf), hh, ww)
+    T.cuda.synchronize()
+    for got, ref_grad in ((gref, r.grad), (gow, o.grad)):
+        g, e = got.cpu().numpy(), ref_grad.numpy()
+        assert np.isfinite(g).all()
+        assert _rel(g, e) <= 2e-6, _rel(g, e)
+
+
 @pytest.mark.parametrize("n_iter", [1, 4])
 def test_pd_iterations_adjoint_matches_autograd(T, n_iter):
     import vfs_hip as H

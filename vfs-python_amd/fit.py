@@ -188,6 +188,7 @@ class ParameterFit:
         return tot / n, (g / n) * th
 
     def step(self, idx):
+        at = self.theta()  # the parameters the step's loss is measured at
         loss, g = self.loss_and_grad(idx)
         self.t += 1
         self.m = self.b1 * self.m + (1 - self.b1) * g
@@ -195,7 +196,7 @@ class ParameterFit:
         mh = self.m / (1 - self.b1 ** self.t)
         vh = self.v / (1 - self.b2 ** self.t)
         self.phi = self.phi - self.lr * mh / (np.sqrt(vh) + self.eps)
-        self.history.append(dict(step=self.t, loss=loss, **self.theta()))
+        self.history.append(dict(step=self.t, loss=loss, at=at, **self.theta()))
         return loss
 
     def run(self, steps, batch, log=None):
@@ -223,14 +224,18 @@ def run_spec(spec, base_seed=3000, log=None):
     hist = f.run(spec["steps"], spec["batch"], log=log)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t1
+    # the iterate with the lowest training loss (the truncated gradient is not the loss's exact
+    # gradient, so the loss need not keep falling); chosen on the training pairs only
     best = min(f.history, key=lambda h: h["loss"])
     theta = f.theta()
     err1, _, hl1 = median_depth_error(held, theta)
+    err_b, _, hl_b = median_depth_error(held, best["at"])
     grads = spec["steps"] * min(spec["batch"], spec["pairs"])
     return dict(spec=spec, start=START, fitted=theta, held_out_median_rel_depth_err_start=err0,
                 held_out_median_rel_depth_err_fitted=err1, held_out_loss_start=hl0, held_out_loss_fitted=hl1,
-                train_loss=[h["loss"] for h in hist], best_step=best["step"], fit_seconds=t_fit,
-                pair_gradients=grads, pair_gradients_per_s=grads / t_fit, generate_seconds=t_gen)
+                train_loss=[h["loss"] for h in hist], best_step=best["step"], best_train_loss=best["loss"],
+                best_theta=best["at"], held_out_median_rel_depth_err_best=err_b, held_out_loss_best=hl_b,
+                fit_seconds=t_fit, pair_gradients=grads, pair_gradients_per_s=grads / t_fit, generate_seconds=t_gen)
 
 
 def main():

@@ -405,11 +405,12 @@ def gradients(items, streams=2, workspaces=None):
         _streams.append(torch.cuda.Stream())
     out = [None] * len(items)
     errors = []
+    caller = torch.cuda.current_stream()  # the inputs are ready in the caller's stream order
 
     def worker(k):
         try:
             st = _streams[k]
-            st.wait_stream(torch.cuda.current_stream())
+            st.wait_stream(caller)
             with torch.cuda.stream(st):
                 for i in range(k, len(items), streams):
                     adj, ref0, other = items[i]
@@ -431,9 +432,8 @@ def gradients(items, streams=2, workspaces=None):
             t.join()
     if errors:
         raise errors[0]
-    cur = torch.cuda.current_stream()
     for k in range(streams):
-        cur.wait_stream(_streams[k])
+        caller.wait_stream(_streams[k])
     for _, g in out:
-        g["ref0"].record_stream(cur)
+        g["ref0"].record_stream(caller)
     return out
