@@ -426,6 +426,10 @@ int vfs_residual_norm_noalloc(void* stream, double* out, int nparts, const float
  * aligned (torch allocations are). */
 size_t vfs_adj_save_bytes(int H, int W, int n_warp, int n_iter, int ckpt);
 size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
+/* Bytes of the optional last-segment tapes of one level (vfs_adj_opts.tape_save): for
+ * each of tape_warps warps, the states after every iteration of the warp's last
+ * checkpoint segment (iterations ckpt (ceil(n_iter / ckpt) - 1) .. n_iter - 1). */
+size_t vfs_adj_tape_bytes(int H, int W, int n_iter, int ckpt, int tape_warps);
 
 /* Options of the adjoint level driver: kernel choices (a zero variant selects the
  * per-kernel path) and the depth of the reverse sweep.
@@ -447,6 +451,12 @@ size_t vfs_adj_workspace_bytes(int H, int W, int ckpt);
  *   reverse_warps: 0 = the exact reverse of every warp; r > 0 = only the last r warps
  *     of the level are reversed and their start state is held fixed (the adjoints of
  *     the level's initial u, u_bar, flow are returned as zero): a truncated gradient;
+ *   tape_save, tape_warps: optional (NULL / 0 = none) tapes of the last checkpoint
+ *     segment of the level's last tape_warps warps (vfs_adj_tape_bytes; 256-byte
+ *     aligned).  The forward writes every state of those segments (it computes them
+ *     anyway) and the reverse reads them instead of recomputing the segment; earlier
+ *     segments are recomputed from their checkpoints as without it.  The gradients are
+ *     the same bits either way.  Both calls of a level must pass the same tapes;
  *   probe_events, probe_units, probe_count, probe_max: an optional timing probe of the
  *     temporally blocked reverse launches (NULL / 0 = none).  For the j-th such launch
  *     of the call, j < probe_max, probe_events[2 j] and probe_events[2 j + 1] (hipEvent_t,
@@ -462,6 +472,8 @@ typedef struct vfs_adj_opts {
     int32_t* probe_units;
     int32_t* probe_count;
     int32_t probe_max;
+    float* tape_save;
+    int32_t tape_warps;
 } vfs_adj_opts;
 
 /* Same results as vfs_solve_level_plain (the per-kernel warp loop of

@@ -320,6 +320,27 @@ def test_gradients_do_not_depend_on_checkpoints_or_kernel_choice(T):
         _assert_same_bits(g0, g)
 
 
+@pytest.mark.parametrize("rw", [0, 1])
+def test_kept_last_segment_tape_gives_the_same_gradients(T, rw):
+    """The forward's tape of each reversed warp's last checkpoint segment (the default)
+    against recomputing that segment: the same loss and gradient bits, for the exact and
+    the truncated gradient, with segments of 4 and 9 iterations (a last segment shorter
+    than the checkpoint interval, and one that is the whole warp, written from the
+    warp's first launch)."""
+    pr, tvf, cvf = _small_case(48, 4)
+    ref0 = torch.as_tensor(O.cast_and_scale(pr["img_ref"]).astype(np.float32), device="cuda")
+    for ck in (4, 9):
+        res = []
+        for keep in (False, True):
+            adj = _adjoint(pr, tvf, cvf, 2, 9, ckpt=ck, reverse_warps=rw, tape_last_segment=keep)
+            assert adj._tape_warps(0) == (0 if not keep else (rw or 2))
+            loss = adj.forward(ref0, pr["img_other"])
+            res.append((loss, adj.backward()))
+        (l0, g0), (l1, g1) = res
+        assert l0 == l1
+        _assert_same_bits(g0, g1)
+
+
 @pytest.mark.parametrize("shape,levels,ratio", [((17, 23), 2, 2.0), ((64, 64), 2, 2.0), ((44, 52), 4, 1.2)])
 def test_gradients_are_bitwise_reproducible(T, shape, levels, ratio):
     """Two gradients of the same pair, back to back on fresh objects, are the same bits:

@@ -43,6 +43,21 @@ inline State state_at(float* base, int64_t n)
 
 inline int n_ckpt(int n_iter, int ck) { return n_iter <= 0 ? 0 : (n_iter + ck - 1) / ck; }
 
+// iterations of the last checkpoint segment (the states a tape of it holds)
+inline int last_seg_len(int n_iter, int ck)
+{
+    const int nck = n_ckpt(n_iter, ck);
+    return nck > 0 ? n_iter - (nck - 1) * ck : 0;
+}
+
+// the last-segment tape of warp w, or nullptr (vfs_adj_opts.tape_save / tape_warps)
+inline float* warp_tape_at(const vfs_adj_opts* o, int64_t n, int n_iter, int ck, int n_warp, int w)
+{
+    if (o == nullptr || o->tape_warps <= 0 || w < n_warp - o->tape_warps) return nullptr;
+    return o->tape_save + static_cast<int64_t>(w - (n_warp - o->tape_warps)) * last_seg_len(n_iter, ck) *
+                              state_floats(n);
+}
+
 // per warp: u, u_bar, flow, then the checkpoints
 inline int64_t warp_rec_floats(int64_t n) { return 2 * al(n) + al(2 * n); }
 inline int64_t warp_save_floats(int64_t n, int n_iter, int ck)
@@ -1101,6 +1116,14 @@ size_t vfs_adj_save_bytes(int H, int W, int n_warp, int n_iter, int ckpt)
     return static_cast<size_t>(n_warp) * static_cast<size_t>(warp_save_floats(n, n_iter, ckpt)) * sizeof(float);
 }
 
+size_t vfs_adj_tape_bytes(int H, int W, int n_iter, int ckpt, int tape_warps)
+{
+    if (H <= 0 || W <= 0 || n_iter < 0 || ckpt < 1 || tape_warps < 0) return 0;
+    const int64_t n = static_cast<int64_t>(H) * W;
+    return static_cast<size_t>(tape_warps) * static_cast<size_t>(last_seg_len(n_iter, ckpt)) *
+           static_cast<size_t>(state_floats(n)) * sizeof(float);
+}
+
 size_t vfs_adj_workspace_bytes(int H, int W, int ckpt)
 {
     if (H <= 0 || W <= 0 || ckpt < 1) return 0;
@@ -1118,6 +1141,9 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
     if (save == nullptr && n_warp > 0) return VFS_EINVAL;
     const int fk = tiles ? tiles->fwd_k : 0;
     if (fk > 0 && workspace == nullptr) return VFS_EINVAL;
+    const int tw = tiles ? tiles->tape_warps : 0;
+    // last-segment tapes are written by the blocked forward only
+    if (tw < 0 || tw > n_warp || (tw > 0 && (tiles->tape_save == nullptr || fk <= 0))) return VFS_EINVAL;
     const int H = lv->H, W = lv->W;
     const int64_t n = static_cast<int64_t>(H) * W;
     hipStream_t s = S(stream);
@@ -1158,10 +1184,48 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
                 int kc = n_iter - done < fk ? n_iter - done : fk;
                 const int to_ck = ckpt - done % ckpt;
                 if (kc > to_ck) kc = to_ck;
-                VFS_TRY(vfs_pd_blocks(stream, H, W, lv->mask, lv->abc, lv->eta, lv->deriv, first ? lv->u : su[cur],
-                                      sub[cur], svb[cur], sp[cur], sq[cur], su[1 - cur], sub[1 - cur], svb[1 - cur],
-                                      sp[1 - cur], sq[1 - cur], first ? 1 : 0, taus + done, sigmas + done, mus + done,
-                                      kc, tiles->fwd_tile_x, tiles->fwd_tile_y, fk, tiles->fwd_threads, prm));
+                const int a_last = (n_ckpt(n_iter, ckpt) - 1) * ckpt;
+                float* wt = warp_tape_at(tiles, n, n_iter, ckpt, n_warp, w);
+                if (wt != nullptr && done >= a_last) {
+                    // the last segment: the same launch, also writing the state after each
+                    // iteration into the warp's tape (slot j: the state after iteration a_last + j)
+                    PdTapeLaunch L;
+                    L.H = H;
+                    L.W = W;
+                    L.mask = lv->mask;
+                    L.abc = lv->abc;
+                    L.eta = lv->eta;
+                    L.deriv = lv->deriv;
+                    const float* ins[5] = {sp[cur], sq[cur], first ? lv->u : su[cur], sub[cur], svb[cur]};
+                    float* outs[5] = {sp[1 - cur], sq[1 - cur], su[1 - cur], sub[1 - cur], svb[1 - cur]};
+                    for (int c = 0; c < 5; ++c) {
+                        L.in[c] = ins[c];
+                        L.out[c] = outs[c];
+                    }
+                    L.taus = taus + done;
+                    L.sigmas = sigmas + done;
+                    L.mus = mus + done;
+                    L.n_iter = kc;
+                    L.tile_x = tiles->fwd_tile_x;
+                    L.tile_y = tiles->fwd_tile_y;
+                    L.k = fk;
+                    L.threads = tiles->fwd_threads;
+                    L.tape = wt + static_cast<int64_t>(done - a_last) * state_floats(n);
+                    L.tape_stride = state_floats(n);
+                    L.tape_off[0] = al(2 * n);
+                    L.tape_off[1] = al(2 * n) + al(4 * n);
+                    L.tape_off[2] = L.tape_off[1] + al(n);
+                    L.tape_off[3] = L.tape_off[2] + al(n);
+                    L.tape_count = kc;
+                    L.first = first ? 1 : 0;
+                    VFS_TRY(pd_blocks_tape(s, L, prm));
+                } else {
+                    VFS_TRY(vfs_pd_blocks(stream, H, W, lv->mask, lv->abc, lv->eta, lv->deriv,
+                                          first ? lv->u : su[cur], sub[cur], svb[cur], sp[cur], sq[cur], su[1 - cur],
+                                          sub[1 - cur], svb[1 - cur], sp[1 - cur], sq[1 - cur], first ? 1 : 0,
+                                          taus + done, sigmas + done, mus + done, kc, tiles->fwd_tile_x,
+                                          tiles->fwd_tile_y, fk, tiles->fwd_threads, prm));
+                }
                 cur = 1 - cur;
                 done += kc;
                 first = false;
@@ -1227,6 +1291,8 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     for (const void* p : ptrs)
         if (p == nullptr) return VFS_EINVAL;
     if (save == nullptr && n_warp > 0) return VFS_EINVAL;
+    const int tw = tiles ? tiles->tape_warps : 0;
+    if (tw < 0 || tw > n_warp || (tw > 0 && tiles->tape_save == nullptr)) return VFS_EINVAL;
     const RevVariant* rv = find_rev_variant(mode);
     if (mode != 0 && mode != 1 && mode != 11 && mode != 12 && mode != 14 && rv == nullptr) return VFS_EINVAL;
     const int H = lv->H, W = lv->W;
@@ -1280,15 +1346,20 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, ws.flow, H, W, lv->tvf, H, W));
         VFS_TRY(vfs_warp_image(stream, lv->warped, ws.flow, H, W, lv->other, H, W));
         VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+        // the forward's tape of this warp's last segment, if it kept one
+        float* wt = warp_tape_at(tiles, n, n_iter, ckpt, n_warp, w);
         // state i of segment seg = [a, b]: the checkpoint for i == a, else tape slot i - a - 1
+        // (of the warp's kept tape for the last segment, of the recompute tape otherwise)
         auto state_of = [&](int seg, int i) {
             const int a = seg * ckpt;
             if (i == a) return state_at(ws.ckpt0 + static_cast<int64_t>(seg) * state_floats(n), n);
-            return state_at(A.tape + static_cast<int64_t>(i - a - 1) * state_floats(n), n);
+            float* tape = (wt != nullptr && seg == nck - 1) ? wt : A.tape;
+            return state_at(tape + static_cast<int64_t>(i - a - 1) * state_floats(n), n);
         };
         // states a + 1 .. b of segment seg into the tape: blocked launches of up to rk
         // iterations that also write their intermediate states, or per-kernel steps
         auto recompute = [&](int seg) {
+            if (wt != nullptr && seg == nck - 1) return 0;  // kept by the forward
             const int a = seg * ckpt, b = (seg + 1) * ckpt < n_iter ? (seg + 1) * ckpt : n_iter;
             if (rtx <= 0) {
                 for (int i = a; i < b; ++i)

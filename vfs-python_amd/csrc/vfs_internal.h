@@ -25,7 +25,8 @@ struct PdTapeLaunch {
     int n_iter, tile_x, tile_y, k, threads;
     float* tape;
     int64_t tape_stride, tape_off[4];
-    int tape_count;
+    int tape_count;  // states after the first tape_count iterations (<= n_iter) into the tape
+    int first = 0;   // 1: a warp's first launch (p = q = v_bar = 0 on input)
 };
 
 int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm);

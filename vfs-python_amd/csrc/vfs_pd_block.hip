@@ -700,7 +700,7 @@ namespace vfs {
 int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm)
 {
     VFS_CHECK_DIMS(a.H, a.W);
-    if (prm == nullptr || a.n_iter < 1 || a.n_iter > a.k || a.tape_count < 0 || a.tape_count >= a.n_iter)
+    if (prm == nullptr || a.n_iter < 1 || a.n_iter > a.k || a.tape_count < 0 || a.tape_count > a.n_iter)
         return VFS_EINVAL;
     if (a.tape_count > 0 && a.tape == nullptr) return VFS_EINVAL;
     const Variant* v = find_variant(a.tile_x, a.tile_y, a.k, a.threads);
@@ -708,7 +708,7 @@ int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm)
     PDLevel L;
     L.H = a.H;
     L.W = a.W;
-    L.first = 0;
+    L.first = a.first ? 1 : 0;
     L.mask = a.mask;
     L.abc = a.abc;
     L.eta = a.eta;

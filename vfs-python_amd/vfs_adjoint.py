@@ -21,7 +21,11 @@ exponent "b") and the level-0 reference image (u8 / 256 as float32).
 Memory: per level, n_warp x (4 + 10 ceil(n_iter / checkpoint)) floats per pixel
 of saves, plus one workspace of (43 + 10 checkpoint) floats per pixel of level 0
 (800 x 800, 10 warps x 50 iterations, checkpoint 25: 0.61 GB of saves for level
-0 and 0.75 GB of workspace).
+0 and 0.75 GB of workspace), plus, by default (``tape_last_segment``), the tape of
+each reversed warp's last checkpoint segment, 10 floats per pixel per iteration of
+that segment (6.4 GB for level 0 of that workload, 8.4 GB for its three levels):
+the forward writes it while computing those states, and the reverse recomputes
+only the earlier segments.
 """
 
 import copy
@@ -54,7 +58,7 @@ def _e(shape, ch=None):
 
 class VFSDepthAdjoint:
     def __init__(self, param, mask, tvf, cvf, cam, t, z_gt, checkpoint=25, fused=True, rows=None, blocked=True,
-                 rec_tile="auto", workspace=None, reverse_warps=0, reverse_block="auto"):
+                 rec_tile="auto", workspace=None, reverse_warps=0, reverse_block="auto", tape_last_segment=True):
         import vfs as V
         self.param = copy.copy(param)
         self.checkpoint = int(checkpoint)
@@ -100,6 +104,11 @@ class VFSDepthAdjoint:
         # 0: exact gradient; r > 0: reverse only the last r warps of each level with their
         # start state held fixed (so only level 0 reaches the loss)
         self.reverse_warps = int(reverse_warps)
+        # the forward keeps the tape of each reversed warp's last checkpoint segment (it
+        # computes those states anyway), so the reverse recomputes only the earlier
+        # segments: n_iter - ckpt more states per warp in HBM, the same gradient bits
+        # (blocked forward only)
+        self.tape_last_segment = bool(tape_last_segment)
         if self.checkpoint < 1:
             raise ValueError("checkpoint must be >= 1")
         self.v = V.VFS(self.param)
@@ -152,19 +161,33 @@ class VFSDepthAdjoint:
         else:
             t.sweep = self._level_mode.get(k, self._mode)
         t.reverse_warps = self.reverse_warps
+        tw = self._tape_warps(k)
+        if tw > 0:
+            t.tape_warps = tw
+            t.tape_save = self._b["tape"][k].data_ptr()
         return t
+
+    def _tape_warps(self, k):
+        """Warps of level k whose last segment's tape the forward keeps: every warp for the
+        exact gradient, the reversed ones of the finest level for the truncated one."""
+        if not (self.tape_last_segment and self.blocked):
+            return 0
+        n_warp = int(self.param.nWarpIters)
+        if self.reverse_warps > 0:
+            return min(self.reverse_warps, n_warp) if k == 0 else 0
+        return n_warp
 
     def _buffers(self):
         """Device buffers for one pair, allocated once (shapes are fixed by the pyramid)."""
         p = self.param
         n_iter, n_warp = int(p.nSolverIters), int(p.nWarpIters)
-        key = (n_iter, n_warp, self.checkpoint)
+        key = (n_iter, n_warp, self.checkpoint, tuple(self._tape_warps(k) for k in range(len(self.sizes))))
         if self._b is not None and self._b["key"] == key:
             return self._b
         L = H.lib()
         sizes = self.sizes
         b = dict(key=key, refs=[_e(sz) for sz in sizes], others=[_e(sz) for sz in sizes], level=[], save=[],
-                 gu=[], gub=[], gf=[], gref=[], gpar=[], other0=None)
+                 gu=[], gub=[], gf=[], gref=[], gpar=[], other0=None, tape=[])
         for k, sz in enumerate(sizes):
             lv = dict(flow=_z(sz, 2), u=_z(sz), u_bar=_z(sz), ref_smoothed=_e(sz), abc=_e(sz, 3), eta=_e(sz, 3),
                       tv=_e(sz, 2), warped=_e(sz), deriv=_e(sz, 2), p=_e(sz, 2), q=_e(sz, 4), tp=_e(sz, 2),
@@ -180,6 +203,8 @@ class VFSDepthAdjoint:
             b["level"].append((lv, st))
             nbytes = L.vfs_adj_save_bytes(sz[0], sz[1], n_warp, n_iter, self.checkpoint)
             b["save"].append(torch.empty(max(nbytes, 256), dtype=torch.uint8, device="cuda"))
+            tb = L.vfs_adj_tape_bytes(sz[0], sz[1], n_iter, self.checkpoint, self._tape_warps(k))
+            b["tape"].append(torch.empty(tb, dtype=torch.uint8, device="cuda") if tb > 0 else None)
             b["gu"].append(_z(sz))
             b["gub"].append(_z(sz))
             b["gf"].append(_z(sz, 2))

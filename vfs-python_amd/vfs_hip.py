@@ -58,7 +58,8 @@ class VfsAdjOpts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("fwd_tile_x", "fwd_tile_y", "fwd_k", "fwd_threads", "rec_tile_x",
                                               "rec_tile_y", "rec_k", "rec_threads", "sweep", "reverse_warps")] + [
         ("probe_events", ctypes.POINTER(ctypes.c_void_p)), ("probe_units", ctypes.POINTER(ctypes.c_int32)),
-        ("probe_count", ctypes.POINTER(ctypes.c_int32)), ("probe_max", ctypes.c_int32)]
+        ("probe_count", ctypes.POINTER(ctypes.c_int32)), ("probe_max", ctypes.c_int32), ("tape_save", c_p),
+        ("tape_warps", ctypes.c_int32)]
 
 
 class VfsAdjPd(ctypes.Structure):
@@ -142,6 +143,7 @@ _SIZE_SIGS = {
     "vfs_batch_workspace_bytes": [c_i],
     "vfs_adj_save_bytes": [c_i, c_i, c_i, c_i, c_i],
     "vfs_adj_workspace_bytes": [c_i, c_i, c_i],
+    "vfs_adj_tape_bytes": [c_i, c_i, c_i, c_i, c_i],
 }
 
 
