@@ -419,8 +419,8 @@ int vfs_residual_norm_noalloc(void* stream, double* out, int nparts, const float
  * synchronisation or memsets (both can be recorded into a HIP graph).
  *
  * Save area of one level (vfs_adj_save_bytes): per warp the warp-start u, u_bar and
- * flow and ceil(n_iter / ckpt) checkpoints of (p, q, u, u_bar, v_bar), taken before
- * iterations 0, ckpt, 2 ckpt, ...  Workspace (vfs_adj_workspace_bytes, one for the
+ * flow, the warp's derivatives (Iu, Iz), and ceil(n_iter / ckpt) checkpoints of
+ * (p, q, u, u_bar, v_bar), taken before iterations 0, ckpt, 2 ckpt, ...  Workspace (vfs_adj_workspace_bytes, one for the
  * largest level serves every level): a tape of ckpt states that a checkpoint
  * segment is recomputed into, plus the adjoint buffers.  Both must be 256-byte
  * aligned (torch allocations are). */
@@ -486,7 +486,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
 
 /* Reverse of vfs_adj_forward_level (opts: see vfs_adj_opts; NULL = per-kernel
  * recompute, sweep 0, every warp).  lv must still hold the forward's ref_smoothed,
- * abc and eta; its tv, warped, deriv, p, q, tp, v_bar, u0, uf are overwritten.
+ * abc and eta; its tv, p, q, tp, v_bar, u0, uf may be overwritten.
  * In: gu, gub (H,W) and gf (H,W,2) are the adjoints of the level's final u, u_bar
  * and flow; on return they are the adjoints of the level's initial u, u_bar and
  * flow.  gref (H,W) += adjoint of the level's reference image; gpar (H,W,5) +=

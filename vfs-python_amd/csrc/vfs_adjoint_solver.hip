@@ -58,15 +58,15 @@ inline float* warp_tape_at(const vfs_adj_opts* o, int64_t n, int n_iter, int ck,
                               state_floats(n);
 }
 
-// per warp: u, u_bar, flow, then the checkpoints
-inline int64_t warp_rec_floats(int64_t n) { return 2 * al(n) + al(2 * n); }
+// per warp: u, u_bar, flow, the warp's derivatives (Iu, Iz), then the checkpoints
+inline int64_t warp_rec_floats(int64_t n) { return 2 * al(n) + 2 * al(2 * n); }
 inline int64_t warp_save_floats(int64_t n, int n_iter, int ck)
 {
     return warp_rec_floats(n) + static_cast<int64_t>(n_ckpt(n_iter, ck)) * state_floats(n);
 }
 
 struct WarpSave {
-    float *u, *ub, *flow;
+    float *u, *ub, *flow, *deriv;
     float* ckpt0;
 };
 
@@ -76,7 +76,8 @@ inline WarpSave warp_save_at(float* save, int64_t n, int n_iter, int ck, int w)
     r.u = save + static_cast<int64_t>(w) * warp_save_floats(n, n_iter, ck);
     r.ub = r.u + al(n);
     r.flow = r.ub + al(n);
-    r.ckpt0 = r.flow + al(2 * n);
+    r.deriv = r.flow + al(2 * n);
+    r.ckpt0 = r.deriv + al(2 * n);
     return r;
 }
 
@@ -218,17 +219,26 @@ __global__ void __launch_bounds__(256) k_save_state(float2* __restrict__ sp, flo
     svb[i] = zero_pqv ? make_float2(0.0f, 0.0f) : vb[i];
 }
 
-// Warp-start record of the blocked path: u, u_bar and flow.
-__global__ void __launch_bounds__(256) k_warp_save(float* __restrict__ su, float* __restrict__ sub,
-                                                   float2* __restrict__ sflow, const float* __restrict__ u,
-                                                   const float* __restrict__ u_bar, const float2* __restrict__ flow,
-                                                   int n)
+// Warp-start record of the blocked path and its first checkpoint: u, u_bar and flow, and
+// the state before iteration 0 (p = q = v_bar = 0, u, u_bar).
+__global__ void __launch_bounds__(256) k_warp_start_save(float* __restrict__ su, float* __restrict__ sub,
+                                                         float2* __restrict__ sflow, float2* __restrict__ cp,
+                                                         float4* __restrict__ cq, float* __restrict__ cu,
+                                                         float* __restrict__ cub, float2* __restrict__ cvb,
+                                                         const float* __restrict__ u, const float* __restrict__ u_bar,
+                                                         const float2* __restrict__ flow, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    su[i] = u[i];
-    sub[i] = u_bar[i];
+    const float ui = u[i], ubi = u_bar[i];
+    su[i] = ui;
+    sub[i] = ubi;
     sflow[i] = flow[i];
+    cp[i] = make_float2(0.0f, 0.0f);
+    cq[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    cu[i] = ui;
+    cub[i] = ubi;
+    cvb[i] = make_float2(0.0f, 0.0f);
 }
 
 // computeOpticalFlowWithClamp with u0 = u in place (the blocked path keeps the
@@ -1160,19 +1170,22 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
         float* sp[2] = {lv->p, t1.p};
         float* sq[2] = {lv->q, t1.q};
         int cur = 0;  // set holding u_bar
+        // the first warp's warping vector and warped image; each warp's end forms the next's
+        VFS_TRY(warp_prep_launch(s, lv->tv, lv->warped, lv->flow, lv->tvf, lv->other, H, W));
         for (int w = 0; w < n_warp; ++w) {
             const WarpSave ws = warp_save_at(save, n, n_iter, ckpt, w);
-            VFS_TRY(vfs_find_warping_vector(stream, lv->tv, lv->flow, H, W, lv->tvf, H, W));
-            VFS_TRY(vfs_warp_image(stream, lv->warped, lv->flow, H, W, lv->other, H, W));
-            VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
-            hipLaunchKernelGGL(k_warp_save, dim3(nb1(n)), dim3(256), 0, s, ws.u, ws.ub,
-                               reinterpret_cast<float2*>(ws.flow), lv->u, sub[cur],
-                               reinterpret_cast<const float2*>(lv->flow), static_cast<int>(n));
+            // the warp's derivatives go to its save record, which the reverse reads again
+            VFS_TRY(vfs_compute_derivatives(stream, ws.deriv, lv->ref, lv->warped, lv->tvf, H, W));
+            const State c0 = state_at(ws.ckpt0, n);
+            hipLaunchKernelGGL(k_warp_start_save, dim3(nb1(n)), dim3(256), 0, s, ws.u, ws.ub,
+                               reinterpret_cast<float2*>(ws.flow), reinterpret_cast<float2*>(c0.p),
+                               reinterpret_cast<float4*>(c0.q), c0.u, c0.ub, reinterpret_cast<float2*>(c0.vb), lv->u,
+                               sub[cur], reinterpret_cast<const float2*>(lv->flow), static_cast<int>(n));
             if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
             int done = 0;
             bool first = true;
             while (done < n_iter) {
-                if (done % ckpt == 0) {
+                if (done > 0 && done % ckpt == 0) {
                     const State c = state_at(ws.ckpt0 + static_cast<int64_t>(done / ckpt) * state_floats(n), n);
                     hipLaunchKernelGGL(k_save_state, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(c.p),
                                        reinterpret_cast<float4*>(c.q), c.u, c.ub, reinterpret_cast<float2*>(c.vb),
@@ -1195,7 +1208,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
                     L.mask = lv->mask;
                     L.abc = lv->abc;
                     L.eta = lv->eta;
-                    L.deriv = lv->deriv;
+                    L.deriv = ws.deriv;
                     const float* ins[5] = {sp[cur], sq[cur], first ? lv->u : su[cur], sub[cur], svb[cur]};
                     float* outs[5] = {sp[1 - cur], sq[1 - cur], su[1 - cur], sub[1 - cur], svb[1 - cur]};
                     for (int c = 0; c < 5; ++c) {
@@ -1220,7 +1233,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
                     L.first = first ? 1 : 0;
                     VFS_TRY(pd_blocks_tape(s, L, prm));
                 } else {
-                    VFS_TRY(vfs_pd_blocks(stream, H, W, lv->mask, lv->abc, lv->eta, lv->deriv,
+                    VFS_TRY(vfs_pd_blocks(stream, H, W, lv->mask, lv->abc, lv->eta, ws.deriv,
                                           first ? lv->u : su[cur], sub[cur], svb[cur], sp[cur], sq[cur], su[1 - cur],
                                           sub[1 - cur], svb[1 - cur], sp[1 - cur], sq[1 - cur], first ? 1 : 0,
                                           taus + done, sigmas + done, mus + done, kc, tiles->fwd_tile_x,
@@ -1231,12 +1244,9 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
                 first = false;
             }
             const float* uth = n_iter > 0 ? su[cur] : lv->u;
-            VFS_TRY(vfs_median_filter(stream, lv->uf, uth, lv->mask, H, W, prm->median_k));
-            hipLaunchKernelGGL(k_flow_clamp_inplace, dim3(nb1(n)), dim3(256), 0, s,
-                               reinterpret_cast<float2*>(lv->flow), lv->u, lv->uf,
-                               reinterpret_cast<const float2*>(lv->tv), lv->mask, static_cast<int>(n),
-                               prm->limit_range);
-            if (hipGetLastError() != hipSuccess) return VFS_EINVAL;
+            // median, clamp, and the next warp's warping vector and warped image
+            VFS_TRY(median_clamp_prep_launch(s, prm->median_k, lv->u, lv->flow, lv->tv, lv->warped, uth, lv->mask,
+                                             lv->tvf, lv->other, H, W, prm->limit_range, w + 1 < n_warp ? 1 : 0));
         }
         if (cur != 0) VFS_TRY(vfs_copy(stream, lv->u_bar, sub[cur], n));
         return 0;
@@ -1245,7 +1255,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
         const WarpSave ws = warp_save_at(save, n, n_iter, ckpt, w);
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, lv->flow, H, W, lv->tvf, H, W));
         VFS_TRY(vfs_warp_image(stream, lv->warped, lv->flow, H, W, lv->other, H, W));
-        VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+        VFS_TRY(vfs_compute_derivatives(stream, ws.deriv, lv->ref, lv->warped, lv->tvf, H, W));
         hipLaunchKernelGGL(k_warp_reset_save, dim3(nb1(n)), dim3(256), 0, s, reinterpret_cast<float2*>(lv->p),
                            reinterpret_cast<float4*>(lv->q), reinterpret_cast<float2*>(lv->v_bar), lv->u0, lv->u,
                            lv->u_bar, reinterpret_cast<const float2*>(lv->flow), ws.u, ws.ub,
@@ -1263,7 +1273,7 @@ int vfs_adj_forward_level(void* stream, const vfs_level* lv, const vfs_params* p
             }
             VFS_TRY(vfs_update_dual(stream, lv->p, lv->tp, lv->q, sigmas[it], lv->u_bar, lv->v_bar, lv->abc, lv->mask,
                                     H, W, prm));
-            VFS_TRY(vfs_l1_thresholding(stream, lv->u, taus[it], lv->tp, lv->u, lv->u_bar, lv->deriv, lv->eta,
+            VFS_TRY(vfs_l1_thresholding(stream, lv->u, taus[it], lv->tp, lv->u, lv->u_bar, ws.deriv, lv->eta,
                                         lv->mask, H, W, prm));
             VFS_TRY(vfs_update_primal(stream, lv->u_bar, lv->v_bar, taus[it], mus[it], lv->p, lv->q, lv->u,
                                       lv->u_bar, lv->v_bar, lv->abc, lv->eta, lv->mask, H, W, prm));
@@ -1343,9 +1353,11 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
     if (probe_max > 0) *tiles->probe_count = 0;
     for (int w = n_warp - 1; w >= w_end; --w) {
         const WarpSave ws = warp_save_at(sv, n, n_iter, ckpt, w);
+        // the warp's warping vector at its start flow; its derivatives are in the save record
         VFS_TRY(vfs_find_warping_vector(stream, lv->tv, ws.flow, H, W, lv->tvf, H, W));
-        VFS_TRY(vfs_warp_image(stream, lv->warped, ws.flow, H, W, lv->other, H, W));
-        VFS_TRY(vfs_compute_derivatives(stream, lv->deriv, lv->ref, lv->warped, lv->tvf, H, W));
+        vfs_level lw = *lv;
+        lw.deriv = ws.deriv;
+        fa.deriv = reinterpret_cast<const float2*>(ws.deriv);
         // the forward's tape of this warp's last segment, if it kept one
         float* wt = warp_tape_at(tiles, n, n_iter, ckpt, n_warp, w);
         // state i of segment seg = [a, b]: the checkpoint for i == a, else tape slot i - a - 1
@@ -1363,7 +1375,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
             const int a = seg * ckpt, b = (seg + 1) * ckpt < n_iter ? (seg + 1) * ckpt : n_iter;
             if (rtx <= 0) {
                 for (int i = a; i < b; ++i)
-                    VFS_TRY(pd_step(stream, lv, prm, state_of(seg, i), state_of(seg, i + 1), A.tp, taus[i],
+                    VFS_TRY(pd_step(stream, &lw, prm, state_of(seg, i), state_of(seg, i + 1), A.tp, taus[i],
                                     sigmas[i], mus[i]));
                 return 0;
             }
@@ -1376,7 +1388,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                 L.mask = lv->mask;
                 L.abc = lv->abc;
                 L.eta = lv->eta;
-                L.deriv = lv->deriv;
+                L.deriv = ws.deriv;
                 const float* ins[5] = {in.p, in.q, in.u, in.ub, in.vb};
                 float* outs[5] = {out.p, out.q, out.u, out.ub, out.vb};
                 for (int c = 0; c < 5; ++c) {
@@ -1506,7 +1518,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                     ba.gpar = gpar;
                     ba.abc = lv->abc;
                     ba.eta = lv->eta;
-                    ba.deriv = reinterpret_cast<const float2*>(lv->deriv);
+                    ba.deriv = reinterpret_cast<const float2*>(ws.deriv);
                     ba.mask = lv->mask;
                     for (int u = 0; u < KRMAX; ++u) {
                         const int iu = u < kc ? i - u : i;  // unused slots repeat the first unit
@@ -1595,7 +1607,7 @@ int vfs_adj_reverse_level(void* stream, const vfs_level* lv, const vfs_params* p
                     ad.v_bar_prev = before.vb;
                     ad.abc = lv->abc;
                     ad.eta = lv->eta;
-                    ad.deriv = lv->deriv;
+                    ad.deriv = ws.deriv;
                     ad.mask = lv->mask;
                     VFS_TRY(vfs_adj_pd_iteration(stream, &ad, prm, taus[it], sigmas[it], mus[it], has_msgs));
                     has_msgs = 1;

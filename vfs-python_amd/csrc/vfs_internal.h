@@ -31,4 +31,13 @@ struct PdTapeLaunch {
 
 int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm);
 
+// The blocked loop's per-warp kernels (vfs_pd_block.hip): the warping vector and warped
+// image at a flow (findWarpingVector + warpingImage), and median + clamp of a warp's
+// thresholded u followed, with prep, by the next warp's warping vector and warped image.
+int warp_prep_launch(hipStream_t s, float* tv, float* warped, const float* flow, const float* tvf, const float* other,
+                     int H, int W);
+int median_clamp_prep_launch(hipStream_t s, int median_k, float* u_ws, float* flow, float* tv, float* warped,
+                             const float* u_th, const uint8_t* mask, const float* tvf, const float* other, int H,
+                             int W, float limit, int prep);
+
 }  // namespace vfs

@@ -734,6 +734,33 @@ int pd_blocks_tape(hipStream_t s, const PdTapeLaunch& a, const vfs_params* prm)
     return launch_pd_single(s, v, L, a.n_iter, sc, make_consts(prm));
 }
 
+int warp_prep_launch(hipStream_t s, float* tv, float* warped, const float* flow, const float* tvf, const float* other,
+                     int H, int W)
+{
+    VFS_CHECK_DIMS(H, W);
+    hipLaunchKernelGGL(k_warp_prep, grid2d(H, W), block2d(), 0, s, reinterpret_cast<float2*>(tv), warped,
+                       reinterpret_cast<const float2*>(flow), reinterpret_cast<const float2*>(tvf), other, H, W);
+    VFS_LAUNCH_STATUS();
+}
+
+int median_clamp_prep_launch(hipStream_t s, int median_k, float* u_ws, float* flow, float* tv, float* warped,
+                             const float* u_th, const uint8_t* mask, const float* tvf, const float* other, int H,
+                             int W, float limit, int prep)
+{
+    VFS_CHECK_DIMS(H, W);
+    if (median_k == 5)
+        hipLaunchKernelGGL(k_median_clamp_prep<5>, grid2d(H, W), block2d(), 0, s, u_ws, reinterpret_cast<float2*>(flow),
+                           reinterpret_cast<float2*>(tv), warped, u_th, mask, reinterpret_cast<const float2*>(tvf),
+                           other, H, W, limit, prep);
+    else if (median_k == 3)
+        hipLaunchKernelGGL(k_median_clamp_prep<3>, grid2d(H, W), block2d(), 0, s, u_ws, reinterpret_cast<float2*>(flow),
+                           reinterpret_cast<float2*>(tv), warped, u_th, mask, reinterpret_cast<const float2*>(tvf),
+                           other, H, W, limit, prep);
+    else
+        return VFS_EINVAL;
+    VFS_LAUNCH_STATUS();
+}
+
 }  // namespace vfs
 
 extern "C" {
