@@ -67,26 +67,25 @@ __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2
                                            bool fwd_y, float a, float b, float c, float2 pc, float4 qc, float scale_p,
                                            float scale_q, float2& pn, float2& tpn, float4& qn)
 {
-    // Forward differences, negated for a backward neighbour: b - a is exactly -(a - b), so
-    // selecting the sign gives the same bits as the two differences written out, without
-    // a divergent branch.
+    // Forward differences, negated for a backward neighbour: b - a is exactly -(a - b).
+    // The sign s = +-1 goes into an fma, s * d - v, whose product is exact, so it rounds
+    // once, as the difference (+-d) - v does; and into the step, (s * scale_q) d + q,
+    // which is scale_q (s d) + q exactly.  Same bits as the differences written out,
+    // without a divergent branch and without a select per component.
     const float dux = uX - uc, dvxx = vX.x - vc.x, dvxy = vX.y - vc.y;
     const float duy = uY - uc, dvyy = vY.y - vc.y, dvyx = vY.x - vc.x;
+    const float sx = fwd_x ? 1.0f : -1.0f, sy = fwd_y ? 1.0f : -1.0f;
     float2 gU;
-    float4 gV;
-    gU.x = (fwd_x ? dux : -dux) - vc.x;
-    gV.x = fwd_x ? dvxx : -dvxx;
-    gV.w = fwd_x ? dvxy : -dvxy;
-    gU.y = (fwd_y ? duy : -duy) - vc.y;
-    gV.y = fwd_y ? dvyy : -dvyy;
-    gV.z = fwd_y ? dvyx : -dvyx;
+    gU.x = fmaf(sx, dux, -vc.x);
+    gU.y = fmaf(sy, duy, -vc.y);
+    const float sqx = sx * scale_q, sqy = sy * scale_q;  // exact: sx, sy are +-1
     const float2 tu = make_float2(fmaf(a, gU.x, c * gU.y), fmaf(c, gU.x, b * gU.y));
     const float2 up = make_float2(fmaf(scale_p, tu.x, pc.x), fmaf(scale_p, tu.y, pc.y));
     const float rp = unit_ball_scale(sq_norm2(up.x, up.y));
     pn = make_float2(up.x * rp, up.y * rp);
     tpn = make_float2(fmaf(a, pn.x, c * pn.y), fmaf(c, pn.x, b * pn.y));
-    const float4 uq = make_float4(fmaf(scale_q, gV.x, qc.x), fmaf(scale_q, gV.y, qc.y), fmaf(scale_q, gV.z, qc.z),
-                                  fmaf(scale_q, gV.w, qc.w));
+    const float4 uq = make_float4(fmaf(sqx, dvxx, qc.x), fmaf(sqy, dvyy, qc.y), fmaf(sqy, dvyx, qc.z),
+                                  fmaf(sqx, dvxy, qc.w));
     const float rq = unit_ball_scale(sq_norm4(uq.x, uq.y, uq.z, uq.w));
     qn = make_float4(uq.x * rq, uq.y * rq, uq.z * rq, uq.w * rq);
 }
