@@ -35,10 +35,12 @@ __device__ __forceinline__ float recip_grad(float d, float eps)
 }
 
 // Scale of the projection onto the unit ball given the squared norm s:
-// 1 / sqrt(s) where s > 1, else 1 (x * 1 is x exactly).
+// 1 / sqrt(s) where s > 1, else 1 (x * 1 is x exactly).  Formed as rsq(max(s, 1)):
+// v_rsq_f32 returns exactly 1 at 1, and fmaxf returns 1 for a NaN s, so this is
+// the select s > 1 ? rsq(s) : 1 bit for bit, in one instruction fewer.
 __device__ __forceinline__ float unit_ball_scale(float s)
 {
-    return s > 1.0f ? __builtin_amdgcn_rsqf(s) : 1.0f;
+    return __builtin_amdgcn_rsqf(fmaxf(s, 1.0f));
 }
 
 __device__ __forceinline__ float sq_norm2(float x, float y)
@@ -93,23 +95,32 @@ __device__ __forceinline__ void dual_pixel(float uc, float2 vc, float uX, float2
 // Linearised-data-term threshold: the prox of lambda |Iu (x - u) + Iz| in the
 // metric eta.x / tau, after the divergence step of T p.  Returns the new u.
 // r0 = recip_eta(eta.x), rIu = recip_grad(Iu).
+// ZeroOut: the caller guarantees tpWx (tpNy) is +0 wherever mW (mN) is false, as it
+// is where every state outside the mask is zero (the blocked kernel); then
+// tc.x - tpWx is tc.x exactly and the mW / mN tests drop out.
+template <bool ZeroOut = false>
 __device__ __forceinline__ float l1_pixel(float2 tc, float tpWx, float tpNy, bool mE, bool mW, bool mS, bool mN,
                                           float dIu, float rIu, float dIz, float r0, float uc, float ubc, float tau,
                                           float lambda)
 {
     float gx, gy;
-    if (mE && mW)
-        gx = tc.x - tpWx;
-    else if (!mE)
-        gx = -tpWx;
-    else
-        gx = tc.x;
-    if (mS && mN)
-        gy = tc.y - tpNy;
-    else if (!mS)
-        gy = -tpNy;
-    else
-        gy = tc.y;
+    if (ZeroOut) {
+        gx = mE ? tc.x - tpWx : -tpWx;
+        gy = mS ? tc.y - tpNy : -tpNy;
+    } else {
+        if (mE && mW)
+            gx = tc.x - tpWx;
+        else if (!mE)
+            gx = -tpWx;
+        else
+            gx = tc.x;
+        if (mS && mN)
+            gy = tc.y - tpNy;
+        else if (!mS)
+            gy = -tpNy;
+        else
+            gy = tc.y;
+    }
     const float tauEtaU = tau * r0;
     const float uTauGtp = ubc + tauEtaU * (gx + gy);
     const float rho = dIu * (uTauGtp - uc) + dIz;
@@ -129,6 +140,10 @@ __device__ __forceinline__ float l1_pixel(float2 tc, float tpWx, float tpNy, boo
 // Primal descent on v and over-relaxation of u and v.  qW / qN are read only
 // when the corresponding neighbour is inside the mask.  r1, r2 = recip_eta(eta.y),
 // recip_eta(eta.z).
+// ZeroOut: the caller guarantees qW (qN) is +0 wherever mW (mN) is false (every
+// state outside the mask is zero, as in the blocked kernel); x - (+0) is x exactly,
+// so qW and qN are subtracted without the tests, with the same bits.
+template <bool ZeroOut = false>
 __device__ __forceinline__ void primal_pixel(float4 qc, float4 qW, float4 qN, bool mE, bool mW, bool mS, bool mN,
                                              float a, float b, float c, float2 pc, float r1, float r2, float2 vb,
                                              float uc, float ub, float tau, float mu, float alpha0, float alpha1,
@@ -139,7 +154,7 @@ __device__ __forceinline__ void primal_pixel(float4 qc, float4 qW, float4 qN, bo
         gQ.x = qc.x;
         gQ.w = qc.w;
     }
-    if (mW) {
+    if (ZeroOut || mW) {
         gQ.x -= qW.x;
         gQ.w -= qW.w;
     }
@@ -147,7 +162,7 @@ __device__ __forceinline__ void primal_pixel(float4 qc, float4 qW, float4 qN, bo
         gQ.y = qc.y;
         gQ.z = qc.z;
     }
-    if (mN) {
+    if (ZeroOut || mN) {
         gQ.y -= qN.y;
         gQ.z -= qN.z;
     }

@@ -310,15 +310,15 @@ __device__ __forceinline__ void pd_tile_run(const PDLevel& L, int tx, int ty, in
             if (f & F_ACT) {
                 const int eW = (f & F_HW) ? e - 1 : e, eN = (f & F_HN) ? e - EX : e;
                 const bool mE = (f & F_ME) != 0, mW = (f & F_MW) != 0, mS = (f & F_MS) != 0, mN = (f & F_MN) != 0;
-                u[j] = l1_pixel(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j], riu[j],
-                                iz[j], r0[j], u[j], ub[j], tau, cst.lambda);
-                // read whether or not the neighbour is inside the mask: primal_pixel uses
-                // qW (qN) only when it is
+                // ZeroOut: a W / N neighbour outside the mask has T p = q = +0 in sB (its p,
+                // q and abc are zero here), which the two steps may then subtract untested
+                u[j] = l1_pixel<true>(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j],
+                                      riu[j], iz[j], r0[j], u[j], ub[j], tau, cst.lambda);
                 const float4 qW = make_float4(sB[2 * E + eW], sB[3 * E + eW], sB[4 * E + eW], sB[5 * E + eW]);
                 const float4 qN = make_float4(sB[2 * E + eN], sB[3 * E + eN], sB[4 * E + eN], sB[5 * E + eN]);
                 float ubn;
                 float2 vbn;
-                primal_pixel(make_float4(qx[j], qy[j], qz[j], qw[j]), qW, qN, mE, mW, mS, mN, ca[j], cb[j], cc[j],
+                primal_pixel<true>(make_float4(qx[j], qy[j], qz[j], qw[j]), qW, qN, mE, mW, mS, mN, ca[j], cb[j], cc[j],
                              make_float2(px[j], py[j]), r1[j], r2[j], make_float2(vbx[j], vby[j]), u[j], ub[j], tau,
                              mu, cst.alpha0, cst.alpha1, ubn, vbn);
                 ub[j] = ubn;
