@@ -265,6 +265,7 @@ __device__ __forceinline__ void pd_tile_run(const PDLevel& L, int tx, int ty, in
     for (int j = 0; j < S; ++j)
         if (skip || !(fl[j] & F_ACT)) ub[j] = vbx[j] = vby[j] = 0.0f;
 
+    float tpx[S], tpy[S];
     for (int k = 0; k < (skip ? 0 : k_count); ++k) {
         const float tau = sc.tau[k], mu = sc.mu[k], sp = sc.sp[k], sq = sc.sq[k];
         __syncthreads();  // sA (u_bar, v_bar) complete
@@ -292,9 +293,12 @@ __device__ __forceinline__ void pd_tile_run(const PDLevel& L, int tx, int ty, in
                 qz[j] = qn.z;
                 qw[j] = qn.w;
             }
-            // T p (dual_pixel's tpn; zero outside the mask, where p and abc are zero)
-            sB[e] = fmaf(ca[j], px[j], cc[j] * py[j]);
-            sB[E + e] = fmaf(cc[j], px[j], cb[j] * py[j]);
+            // T p (dual_pixel's tpn; zero outside the mask, where p and abc are zero), kept
+            // in registers for the pixel's own threshold step
+            tpx[j] = fmaf(ca[j], px[j], cc[j] * py[j]);
+            tpy[j] = fmaf(cc[j], px[j], cb[j] * py[j]);
+            sB[e] = tpx[j];
+            sB[E + e] = tpy[j];
             sB[2 * E + e] = qx[j];
             sB[3 * E + e] = qy[j];
             sB[4 * E + e] = qz[j];
@@ -312,7 +316,7 @@ __device__ __forceinline__ void pd_tile_run(const PDLevel& L, int tx, int ty, in
                 const bool mE = (f & F_ME) != 0, mW = (f & F_MW) != 0, mS = (f & F_MS) != 0, mN = (f & F_MN) != 0;
                 // ZeroOut: a W / N neighbour outside the mask has T p = q = +0 in sB (its p,
                 // q and abc are zero here), which the two steps may then subtract untested
-                u[j] = l1_pixel<true>(make_float2(sB[e], sB[E + e]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j],
+                u[j] = l1_pixel<true>(make_float2(tpx[j], tpy[j]), sB[eW], sB[E + eN], mE, mW, mS, mN, iu[j],
                                       riu[j], iz[j], r0[j], u[j], ub[j], tau, cst.lambda);
                 const float4 qW = make_float4(sB[2 * E + eW], sB[3 * E + eW], sB[4 * E + eW], sB[5 * E + eW]);
                 const float4 qN = make_float4(sB[2 * E + eN], sB[3 * E + eN], sB[4 * E + eN], sB[5 * E + eN]);
