@@ -97,7 +97,8 @@ __device__ __forceinline__ void gather_msgs(const uint8_t* __restrict__ mask, Ms
 
 // Reverse of updatePrimalVariables (Python/vfs.cu:507-570) at a masked-in pixel.
 // g_ubo / g_vbo: complete adjoints of u_bar_t / v_bar_t.  State: q_t at the pixel
-// and its W / N neighbours (zero where masked), p_t, abc, eta.y, eta.z.
+// and its W / N neighbours (+0 where that neighbour is outside the mask, so that
+// x - qW is x there and the forward's gQ needs no test), p_t, abc, eta.y, eta.z.
 // Sets gubB / gvbB (own part of the adjoints of u_bar_{t-1} / v_bar_{t-1}), qmsg;
 // adds to gu (adjoint of u_t), gp, gq, gabc, geta.y/.z, and the alpha shares.
 __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4 qW, float4 qN, float a, float b,
@@ -112,18 +113,14 @@ __device__ __forceinline__ void primal_adj_pixel(const Nbr& n, float4 qc, float4
         gQ.x = qc.x;
         gQ.w = qc.w;
     }
-    if (n.mW) {
-        gQ.x -= qW.x;
-        gQ.w -= qW.w;
-    }
+    gQ.x -= qW.x;
+    gQ.w -= qW.w;
     if (n.mS) {
         gQ.y = qc.y;
         gQ.z = qc.z;
     }
-    if (n.mN) {
-        gQ.y -= qN.y;
-        gQ.z -= qN.z;
-    }
+    gQ.y -= qN.y;
+    gQ.z -= qN.z;
     const float2 tu = make_float2(fmaf(a, pc.x, c * pc.y), fmaf(c, pc.x, b * pc.y));
     const bool d1 = !(fabsf(e1) < kEps), d2 = !(fabsf(e2) < kEps);
     const float r1 = recip_eta(e1, kEps), r2 = recip_eta(e2, kEps);
@@ -178,7 +175,8 @@ __device__ __forceinline__ float2 tp_at(const float2* __restrict__ p, const floa
 
 // Reverse of l1Thresholding (Python/vfs.cu:421-495) at a masked-in pixel.
 // gout: complete adjoint of u_t.  tc, tpWx, tpNy: T p_t at the pixel, x of its W
-// neighbour, y of its N neighbour.  State: u_{t-1}, u_bar_{t-1}, deriv, eta.x.
+// neighbour, y of its N neighbour (+0 where that neighbour is outside the mask, so
+// that tc.x - tpWx is tc.x there).  State: u_{t-1}, u_bar_{t-1}, deriv, eta.x.
 // Returns the adjoint of u_{t-1}; sets tmsg = (own tp.x, own tp.y, to W (x), to N (y));
 // adds to gubB, geta.x, gderiv and the lambda share.
 __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 tc, float tpWx, float tpNy, float2 d,
@@ -206,8 +204,8 @@ __device__ __forceinline__ float l1_adj_pixel(const Nbr& n, float gout, float2 t
         cyc = 1.0f;
         cyn = 0.0f;
     }
-    const float gx = (n.mE && n.mW) ? tc.x - tpWx : (!n.mE ? -tpWx : tc.x);
-    const float gy = (n.mS && n.mN) ? tc.y - tpNy : (!n.mS ? -tpNy : tc.y);
+    const float gx = n.mE ? tc.x - tpWx : -tpWx;
+    const float gy = n.mS ? tc.y - tpNy : -tpNy;
     const float dIu = d.x, dIz = d.y;
     const bool de = !(fabsf(e0) < kEps);
     const float r0 = recip_eta(e0, kEps);
@@ -353,10 +351,10 @@ __device__ __forceinline__ void dual_adj_pixel(bool fx, bool fy, float uc, float
     const float duy = uY - uc, dvyy = vY.y - vc.y, dvyx = vY.x - vc.x;
     float2 gU;
     float4 gV;
-    gU.x = (fx ? dux : -dux) - vc.x;
+    gU.x = fmaf(sx, dux, -vc.x);  // (+-dux) - vc.x: the product is exact, one rounding
     gV.x = fx ? dvxx : -dvxx;
     gV.w = fx ? dvxy : -dvxy;
-    gU.y = (fy ? duy : -duy) - vc.y;
+    gU.y = fmaf(sy, duy, -vc.y);
     gV.y = fy ? dvyy : -dvyy;
     gV.z = fy ? dvyx : -dvyx;
     const float sp = st.sp, sq = st.sq;
