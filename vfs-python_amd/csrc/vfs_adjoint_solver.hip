@@ -733,14 +733,21 @@ __global__ void __launch_bounds__(NT) k_adj_block(const BlockRevArgs a)
             gi[j] = idx;
             f |= RB_IMG;
             if (ex >= KR && ex < KR + TX && ey >= KR && ey < KR + TY) f |= RB_TILE;
-            if (a.mask[idx] != 0) {
+            // the pixel's and its four neighbours' mask bytes in one round trip to memory (the
+            // empty asm takes them, so that the neighbours' loads are not put behind the test of
+            // the pixel's own)
+            const unsigned m0 = a.mask[idx], mEb = a.mask[gy * W + clamp_idx(gx + 1, W)];
+            const unsigned mSb = a.mask[clamp_idx(gy + 1, H) * W + gx], mWb = a.mask[gy * W + clamp_idx(gx - 1, W)];
+            const unsigned mNb = a.mask[clamp_idx(gy - 1, H) * W + gx];
+            asm volatile("" ::"v"(m0), "v"(mEb), "v"(mSb), "v"(mWb), "v"(mNb));
+            if (m0 != 0) {
                 f |= RB_M;
-                const bool mE = a.mask[gy * W + clamp_idx(gx + 1, W)] != 0;
-                const bool mS = a.mask[clamp_idx(gy + 1, H) * W + gx] != 0;
+                const bool mE = mEb != 0;
+                const bool mS = mSb != 0;
                 if (mE) f |= RB_ME;
-                if (a.mask[gy * W + clamp_idx(gx - 1, W)] != 0) f |= RB_MW;
+                if (mWb != 0) f |= RB_MW;
                 if (mS) f |= RB_MS;
-                if (a.mask[clamp_idx(gy - 1, H) * W + gx] != 0) f |= RB_MN;
+                if (mNb != 0) f |= RB_MN;
                 // the neighbour whose u_bar, v_bar this pixel's dual step read (dual_x_target)
                 if (mE ? gx + 1 < W : gx > 0) f |= mE ? RB_TXP : RB_TXM;
                 if (mS ? gy + 1 < H : gy > 0) f |= mS ? RB_TYP : RB_TYM;
